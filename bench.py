@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-ray hot path on one MI355X.
+
+Workload: 256 x 256 image, a = 0.5, inclination 30 deg, 230 GHz, through a
+synthetic analytic torus stored as 512 meshblocks of 32^3 FP32 cells over
++-40 M (a uniform 256^3 grid).  A step is one image: one launch of the fused
+kernel (camera ray, RK4 geodesic, sampling, emission, transfer) over all
+65 536 rays, inputs resident in HBM.  The timed precision is the mixed
+FP32/FP64 kernel; the plain FP64 kernel is timed the same way beside it.
+
+Prints one JSON line.  rays/s counts camera rays fully integrated.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MSUN = 1.989e33
+FLOP_PER_STEP = 1000.0        # algorithmic FLOP per accepted RK4 ray-step
+PEAK_FP64 = 78.6              # TFLOP/s, MI355X FP64 vector (datasheet)
+PEAK_FP32 = 157.3             # TFLOP/s, MI355X FP32 vector (datasheet)
+HBM_PEAK = 8000.0             # GB/s
+
+
+def ring_diameter(img, fov):
+    """Twice the radius of the peak of the azimuthally averaged profile."""
+    res = img.shape[0]
+    c = np.linspace(-fov / 2, fov / 2, 2 * res + 1)[1::2]
+    x, y = np.meshgrid(c, c, indexing="ij")
+    r = np.hypot(x, y)
+    bins = np.linspace(0, fov / 2, res // 2 + 1)
+    idx = np.digitize(r.ravel(), bins) - 1
+    ok = (idx >= 0) & (idx < res // 2)
+    prof = np.bincount(idx[ok], img.ravel()[ok], res // 2) / np.maximum(np.bincount(idx[ok], None, res // 2), 1)
+    k = int(np.argmax(prof))
+    return float(bins[k] + bins[k + 1])
+
+
+def time_launches(L, fn, steps, warmup):
+    """Per-launch durations (ms) from HIP events on the launch stream, plus
+    the wall time of the whole timed region bracketed by device syncs."""
+    import ctypes as C
+    for _ in range(warmup):
+        fn()
+    L.check(L.lib().mk_stream_sync(None))
+    evs = []
+    for _ in range(2 * steps):
+        e = C.c_void_p()
+        L.check(L.lib().mk_event_create(C.byref(e)))
+        evs.append(e.value)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        L.check(L.lib().mk_event_record(evs[2 * s], None))
+        fn()
+        L.check(L.lib().mk_event_record(evs[2 * s + 1], None))
+    L.check(L.lib().mk_stream_sync(None))
+    wall = time.perf_counter() - t0
+    ms = []
+    for s in range(steps):
+        v = C.c_float()
+        L.check(L.lib().mk_event_elapsed_ms(evs[2 * s], evs[2 * s + 1], C.byref(v)))
+        ms.append(v.value)
+    for e in evs:
+        L.lib().mk_event_destroy(e)
+    return np.array(ms), wall
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--res", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--only", choices=["mixed", "plain64"], default=None,
+                    help="time one precision only (profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from mahakala_amd import _lib as L
+    from mahakala_amd import images
+    from mahakala_amd.grmhd.athenak import AthenakFluidModel
+    from mahakala_amd.grmhd.synthetic import config2_snapshot
+
+    a, inc, nu, fov, res = 0.5, 30.0, 230e9, 20.0, args.res
+    t0 = time.perf_counter()
+    ds = config2_snapshot(a, seed=0)
+    model = AthenakFluidModel.from_datasets(ds, a, fluid_gamma=13. / 9)
+    snap = model.device_snapshot
+    print(f"[bench] snapshot ready in {time.perf_counter() - t0:.1f}s ({snap.nbytes / 2**20:.0f} MiB on device)",
+          file=sys.stderr, flush=True)
+    n = res * res
+    img = L.DeviceBuffer(8 * n)
+    stp = L.DeviceBuffer(4 * n)
+    par = {p: images.render_params(a, camera_inclination=inc, fov=fov, resolution=res, observing_frequency=nu,
+                                   precision=p) for p in ("plain64", "mixed")}
+
+    def launch(p, steps_out=False):
+        def fn():
+            L.check(L.lib().mk_render(snap.handle, par[p], img.ptr, stp.ptr if steps_out else None, None))
+        return fn
+
+    # reference run for accepted steps and accuracy (untimed)
+    launch("plain64", True)()
+    L.sync()
+    I_plain = img.to_array((res, res))
+    steps_plain = stp.to_array((res, res), np.int32)
+    launch("mixed")()
+    L.sync()
+    I_mixed = img.to_array((res, res))
+    total_steps = float(steps_plain.sum())
+
+    results = {}
+    for p in ("plain64", "mixed"):
+        if args.only and p != args.only:
+            continue
+        ms, wall = time_launches(L, launch(p), args.steps, args.warmup)
+        results[p] = dict(ms=ms, wall=wall)
+        print(f"[bench] {p}: {np.mean(ms):.3f} ms/image (p50 {np.percentile(ms, 50):.3f}, "
+              f"p99 {np.percentile(ms, 99):.3f}), wall {wall:.3f}s for {args.steps}", file=sys.stderr, flush=True)
+
+    flux_p, flux_m = float(I_plain.sum()), float(I_mixed.sum())
+    acc = dict(flux_rel_err=abs(flux_m - flux_p) / flux_p,
+               pixel_l1_rel=float(np.abs(I_mixed - I_plain).sum() / I_plain.sum()),
+               ring_diameter_plain=ring_diameter(I_plain, fov), ring_diameter_mixed=ring_diameter(I_mixed, fov))
+
+    head = args.only or "mixed"
+    ms_head = results[head]["ms"]
+    wall = results[head]["wall"]
+    value = n * args.steps / wall
+    kern_ms = float(np.mean(ms_head))
+    achieved = FLOP_PER_STEP * total_steps / (kern_ms * 1e-3) / 1e12
+    peak = PEAK_FP32 if head == "mixed" else PEAK_FP64
+    roofline = dict(bound="valu", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
+                    frac=round(achieved / peak, 4), traffic=None,
+                    kernel=f"k_render<{'true' if head == 'mixed' else 'false'}>",
+                    algorithmic=f"{FLOP_PER_STEP:.0f} FLOP x {total_steps:.0f} accepted ray-steps per launch")
+    tfile = os.path.join(ROOT, "profiles", "render_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tr = json.load(f).get(head)
+        if tr:
+            roofline["traffic"] = tr["bytes_per_launch"]
+
+    out = {
+        "metric": "rays/s (256^2 Kerr + RT), mixed vs plain FP64 at 1e-3 flux err; p50/p99 latency",
+        "value": round(value, 1), "unit": "rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32+fp64" if head == "mixed" else "fp64", "data": "synthetic",
+        "config": {"workload": "256x256 image, a=0.5, i=30deg, 230 GHz, analytic torus on 512 meshblocks of 32^3 "
+                               "FP32 cells over +-40 M (configs[1])",
+                   "resolution": res, "rays_per_step": n, "precision": head, "max_nsteps": 10000},
+        "latency_ms": {"p50": round(float(np.percentile(ms_head, 50)), 4),
+                       "p99": round(float(np.percentile(ms_head, 99)), 4), "what": "one 256^2 image per launch"},
+        "roofline": roofline,
+        "accuracy_mixed_vs_plain": {k: (round(v, 9) if isinstance(v, float) else v) for k, v in acc.items()},
+        "accepted_steps": int(total_steps), "mean_steps_per_ray": round(total_steps / n, 2),
+        "max_steps_per_ray": int(steps_plain.max()),
+    }
+    if "plain64" in results:
+        pw = results["plain64"]["wall"]
+        pms = float(np.mean(results["plain64"]["ms"]))
+        a64 = FLOP_PER_STEP * total_steps / (pms * 1e-3) / 1e12
+        out["plain_fp64"] = {"value": round(n * args.steps / pw, 1), "unit": "rays/s", "ms_per_image": round(pms, 4),
+                             "achieved_tflops": round(a64, 3), "peak": PEAK_FP64, "frac": round(a64 / PEAK_FP64, 4)}
+        if "mixed" in results:
+            out["speedup_mixed_over_plain"] = round(pw / results["mixed"]["wall"], 3)
+
+    if not args.no_cpu:
+        from oracle import c_oracle
+        from tests.helpers import c_snapshot_from_model
+        cs = c_snapshot_from_model(model)
+        nth = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t1 = time.perf_counter()
+        I_cpu, st_cpu = cs.render(a, inc, 1000.0, fov, res, 10000, 13. / 9, 40.0, 1e26, 6.2e9 * MSUN, nu,
+                                  nthreads=nth)
+        dt_cpu = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": round(n / dt_cpu, 1), "unit": "rays/s", "cores": nth, "kind": "port",
+                               "sample": f"the full {res}x{res} image once, C FP64 restatement with OpenMP "
+                                         f"({dt_cpu:.1f} s wall)",
+                               "flux_rel_err_vs_gpu_plain": abs(float(I_cpu.sum()) - flux_p) / flux_p}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

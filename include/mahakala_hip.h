@@ -1,0 +1,185 @@
+/*
+ * mahakala_hip.h -- C ABI of libmahakala_hip.so, the MI355X (gfx950) kernels
+ * behind Mahakala's per-ray hot path.
+ *
+ * Conventions
+ *   - Every entry point returns int: MK_OK (0) or a negative MK_ERR_* code;
+ *     mk_last_error() gives the thread's last message.  Nothing throws.
+ *   - Pointers named d_* are device pointers (mk_malloc or any HIP
+ *     allocation on the current device); h_* are host pointers.  Arrays are
+ *     C-contiguous FP64 unless stated.  `stream` is a hipStream_t (NULL for
+ *     the default stream); calls are asynchronous on it unless they return
+ *     host results, which synchronise it.
+ *   - Layouts follow the reference: states (n, 8) = [t, x, y, z, k^t, k^x,
+ *     k^y, k^z]; histories (T, n, 8) and (T, n); images pixel p = i*res + j
+ *     with image x = grid[i], image y = grid[j] (meshgrid 'ij').
+ *   - NaN semantics are the reference's and stay in-band.
+ *
+ * Each declaration names the reference function it stands in for
+ * (file:line in the reference's mahakala/ package).
+ */
+#ifndef MAHAKALA_HIP_H
+#define MAHAKALA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MK_OK 0
+#define MK_ERR_ARG (-1)     /* bad argument (shape, range, null pointer)      */
+#define MK_ERR_HIP (-2)     /* HIP runtime error                              */
+#define MK_ERR_MESH (-3)    /* inconsistent meshblock tree (athenak.py:190)   */
+#define MK_ERR_NOMEM (-4)   /* allocation failed or over the memory budget    */
+#define MK_ERR_STATE (-5)   /* object in the wrong state (closed server, ...) */
+#define MK_ERR_TIMEOUT (-6) /* wait timed out                                 */
+
+/* ------------------------------------------------------------------ runtime */
+const char* mk_last_error(void);
+int mk_version(void);
+/* device name, compute units and HBM size of the current device */
+int mk_device_info(char* h_name, size_t name_len, int* h_cus, size_t* h_total_bytes);
+int mk_malloc(void** d_ptr, size_t bytes);
+int mk_free(void* d_ptr);
+int mk_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes, void* stream);
+int mk_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes, void* stream);
+int mk_memset(void* d_dst, int value, size_t bytes, void* stream);
+int mk_stream_create(void** stream);
+int mk_stream_destroy(void* stream);
+int mk_stream_sync(void* stream);
+int mk_event_create(void** event);
+int mk_event_destroy(void* event);
+int mk_event_record(void* event, void* stream);
+int mk_event_sync(void* event);
+int mk_event_elapsed_ms(void* start, void* stop, float* h_ms);
+
+/* ------------------------------------------------------------------- camera */
+/* get_initial_grid('grid') + initial_condition  (geodesics.py:29-55, 137-181,
+ * 204-230): res*res null camera rays -> d_out (res*res, 8). */
+int mk_camera_grid(double bhspin, double inclination_deg, double distance, double fov_lower, double fov_upper,
+                   int res, double* d_out, void* stream);
+/* get_camera_pixel + initial_condition  (geodesics.py:107-134, 219-230) for
+ * polar image-plane points; h_radius, h_angle (n,) host. */
+int mk_camera_pixels(double bhspin, double inclination_deg, double distance, const double* h_radius,
+                     const double* h_angle, long n, double* d_out, void* stream);
+/* initial_condition / _Nullify  (geodesics.py:58-85, 219-230): positions
+ * d_x (n, 4) and directions d_v (n, 4) -> null states d_out (n, 8). */
+int mk_initial_condition(const double* d_x, const double* d_v, long n, double bhspin, double* d_out, void* stream);
+
+/* ------------------------------------------------------------------- metric */
+/* metric  (geodesics.py:88-104): d_x (n, 4) -> d_g (n, 4, 4). */
+int mk_metric(const double* d_x, long n, double bhspin, double* d_g, void* stream);
+/* imetric  (geodesics.py:339-347): closed form eta - f l l -> (n, 4, 4). */
+int mk_imetric(const double* d_x, long n, double bhspin, double* d_g, void* stream);
+/* rhs / _vectorized_rhs  (geodesics.py:294-314): d_s (n, 8) -> d_out (n, 8). */
+int mk_rhs(const double* d_s, long n, double bhspin, double* d_out, void* stream);
+/* RK4_gen  (geodesics.py:317-336): d_s (n, 8), d_dt (n,) -> d_out (n, 8). */
+int mk_rk4(const double* d_s, const double* d_dt, long n, double bhspin, double* d_out, void* stream);
+/* radius_cal  (geodesics.py:284-291): points with `stride` doubles per point
+ * (x at offsets 1..3) -> d_r (n,). */
+int mk_radius_cal(const double* d_x, long n, int stride, double bhspin, double* d_r, void* stream);
+
+/* --------------------------------------------------------------- integrator */
+/* geodesic_integrator  (geodesics.py:233-281), final-state form: per ray the
+ * first frozen row K (N if it never froze), the state of its last emitted
+ * row (n, 8), and the value select_photons_integrator reads for it
+ * (geodesics.py:370-378).  Any output may be NULL. */
+int mk_integrate_final(const double* d_s0, long n, int N, double div, double tol, double bhspin, int32_t* d_K,
+                       double* d_last, double* d_select_r, void* stream);
+/* trim rule (geodesics.py:275-281): T from per-ray K -> *h_T. */
+int mk_trim_rows(const int32_t* d_K, long n, int N, long* h_T, void* stream);
+/* geodesic_integrator, materialised form: rows 0..T-1 into d_S (T, n, 8)
+ * and d_dt (T, n). */
+int mk_integrate_history(const double* d_s0, long n, int N, double div, double tol, double bhspin, long T,
+                         double* d_S, double* d_dt, void* stream);
+
+/* ----------------------------------------------------------------- shadow */
+/* select_photons_integrator  (geodesics.py:354-378): final radius per
+ * (radius, angle) camera point; host in, host out. */
+int mk_select_photons(double inclination_deg, const double* h_angle, const double* h_radius, long n, double bhspin,
+                      double distance, int max_steps, double* h_out);
+/* find_shadow_bisection_angles  (geodesics.py:405-435): host in, host out. */
+int mk_shadow_bisect(double bhspin, double inclination_deg, const double* h_angles, long n, int max_steps,
+                     double error_allowed, int max_it, double distance, double* h_radii);
+
+/* ---------------------------------------------------------------- transfer */
+/* synchrotron_coefficients  (transfer.py:30-86), elementwise over n. */
+int mk_synchrotron(const double* d_Ne, const double* d_Theta_e, const double* d_B, const double* d_pitch,
+                   const double* d_nu, long n, int invariant, double rescale_nu, double* d_j, double* d_alpha,
+                   void* stream);
+/* solve_specific_intensity  (transfer.py:89-119): (T, npx) inputs ->
+ * d_I (npx,); d_dI (T-1, npx) in scan order when non-NULL (dIs=True). */
+int mk_solve_intensity(const double* d_j, const double* d_alpha, const double* d_dt, long T, long npx, double L_unit,
+                       double* d_I, double* d_dI, void* stream);
+/* solve_attenuated_emissivity  (transfer.py:122-144) -> d_out (T-1, npx). */
+int mk_attenuated_emissivity(const double* d_j, const double* d_alpha, const double* d_dt, long T, long npx,
+                             double L_unit, double* d_out, void* stream);
+/* rlow_rhigh_model  (electrons.py:32-50), elementwise. */
+int mk_rlow_rhigh(const double* d_dens, const double* d_u, const double* d_beta, long n, double r_low, double r_high,
+                  double electron_gamma, double ion_gamma, double* d_Theta_e, void* stream);
+
+/* ---------------------------------------------------------------- snapshot */
+typedef struct mk_snapshot mk_snapshot;
+
+/* AthenakFluidModel.load_athenak_meshblocks  (grmhd/athenak.py:67-525) from
+ * in-memory datasets: h_prims (8, nmb, n, n, n) float32 holds uov then B in
+ * file variable order; h_var_index[8] gives the row of dens, eint, velx,
+ * vely, velz, bcc1, bcc2, bcc3 (get_index_for_primitive_by_name,
+ * athenak.py:55-65); h_levels (nmb,), h_logical (nmb, 3) as Levels and
+ * LogicalLocations; h_x{1,2,3}f (nmb, n+1) and h_x{1,2,3}v (nmb, n).  Builds
+ * the ghost-padded FP32 layout on the device (same-level copy, coarser
+ * injection, finer 2x2x2 restriction) and an O(1) leaf lookup grid. */
+int mk_snapshot_create(const float* h_prims, const int32_t* h_var_index, long nmb, int n, const int32_t* h_levels,
+                       const int32_t* h_logical, const double* h_x1f, const double* h_x2f, const double* h_x3f,
+                       const double* h_x1v, const double* h_x2v, const double* h_x3v, mk_snapshot** out);
+int mk_snapshot_free(mk_snapshot* snap);
+/* device bytes held, and the ghost-padded cells as (nmb, n+2, n+2, n+2, 8)
+ * float32 in canonical order [rho, u, U1, U2, U3, B1, B2, B3] */
+int mk_snapshot_bytes(const mk_snapshot* snap, size_t* h_bytes);
+int mk_snapshot_download(const mk_snapshot* snap, float* h_cells);
+/* meshblock id per point (athenak.py:657-670), -1 outside; d_S (n, stride) */
+int mk_block_index(const mk_snapshot* snap, const double* d_S, long n, int stride, int32_t* d_mb, void* stream);
+/* get_prims_from_geodesics  (athenak.py:527-637): d_S (n, 8) -> d_out (8, n)
+ * in canonical order, zero outside the domain. */
+int mk_sample_prims(const mk_snapshot* snap, const double* d_S, long n, double* d_out, void* stream);
+/* get_fluid_scalars_from_geodesics  (athenak.py:639-812): d_S (n, 8) ->
+ * d_out (5, n) = dens, u, pitch_angle, kdotu, b. */
+int mk_sample_scalars(const mk_snapshot* snap, const double* d_S, long n, double bhspin, double fallback_pitch_angle,
+                      double* d_out, void* stream);
+
+/* ----------------------------------------------------------- fused render */
+typedef struct mk_render_params {
+  double bhspin;              /* a (overrides the snapshot's spin)          */
+  double inclination;         /* camera inclination, degrees                */
+  double distance;            /* camera distance, GM/c^2                    */
+  double fov;                 /* full field of view, GM/c^2                 */
+  double mass_scale;          /* g                                          */
+  double M_bh;                /* g                                          */
+  double r_high;              /* R(beta) model                              */
+  double fluid_gamma;         /* adiabatic index of the fluid               */
+  double observing_frequency; /* Hz                                         */
+  double div;                 /* step divisor (40)                          */
+  double tol;                 /* horizon tolerance (1e-4)                   */
+  double fallback_pitch;      /* pi/3                                       */
+  double sigma_cut;           /* 100                                        */
+  double fp64_radius;         /* mixed: FP64 substeps where r < this        */
+  double ring_radius;         /* mixed: FP64 where r < this and             */
+  double ring_drdl;           /*        |dr/dlambda| < this                 */
+  int32_t resolution;         /* pixels per side                            */
+  int32_t max_nsteps;         /* N (10000)                                  */
+  int32_t precision;          /* 0 = plain FP64, 1 = mixed FP32/FP64        */
+  int32_t flags;              /* reserved, 0                                */
+} mk_render_params;
+
+/* make_image  (images.py:30-144) for a device-resident snapshot: camera,
+ * geodesic, sampling, emission and transfer fused per ray; d_image
+ * (res*res,) FP64, d_steps (res*res,) accepted steps per ray (may be NULL). */
+int mk_render(const mk_snapshot* snap, const mk_render_params* params, double* d_image, int32_t* d_steps,
+              void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAHAKALA_HIP_H */

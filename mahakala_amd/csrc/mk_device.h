@@ -1,0 +1,492 @@
+// mk_device.h -- device-side physics of the per-ray hot path (gfx950).
+//
+// Everything here runs in registers of one lane per ray.  The arithmetic
+// follows the reference (file:line in mahakala/):
+//   Kerr-Schild metric            geodesics.py:88-104
+//   geodesic right-hand side      geodesics.py:294-309  (jacfwd + LU there;
+//                                 here the closed-form gradient of the same
+//                                 expression and g^{mu nu} = eta - f l^mu l^nu)
+//   RK4                           geodesics.py:317-336
+//   radius, step rule             geodesics.py:249-252, 284-291
+//   camera + nullify              geodesics.py:58-85, 107-134, 204-216
+//   sampler                       grmhd/athenak.py:712-794
+//   electrons, synchrotron        electrons.py:32-50, transfer.py:30-86
+//   per-sample glue               images.py:86-118
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mk {
+
+// constants.py:23-31 (electrons.py and grmhd.py repeat MP)
+constexpr double kEE = 4.8032e-10;
+constexpr double kCL = 2.99792458e10;
+constexpr double kME = 9.1094e-28;
+constexpr double kMP = 1.6726e-24;
+constexpr double kHPL = 6.6261e-27;
+constexpr double kPI = 3.141592653589793;
+
+// ---------------------------------------------------------------- precision
+// Plain FP64 uses IEEE division and square root.  The FP32 path uses the
+// hardware reciprocal / square root (about 1 ulp) and exp2/log2.
+template <typename T> struct Math;
+template <> struct Math<double> {
+  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+  static __device__ __forceinline__ double sqrt(double x) { return __builtin_sqrt(x); }
+  static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+  static __device__ __forceinline__ double pow(double x, double y) { return ::pow(x, y); }
+  static __device__ __forceinline__ double sin(double x) { return ::sin(x); }
+  static __device__ __forceinline__ double acos(double x) { return ::acos(x); }
+  static __device__ __forceinline__ double cos(double x) { return ::cos(x); }
+  static __device__ __forceinline__ double abs(double x) { return __builtin_fabs(x); }
+};
+template <> struct Math<float> {
+  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+  static __device__ __forceinline__ float pow(float x, float y) {
+    // x >= 0 here; log2(0) = -inf gives 0, log2(inf) = inf gives inf
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+  }
+  static __device__ __forceinline__ float sin(float x) { return ::__sinf(x); }
+  static __device__ __forceinline__ float acos(float x) { return ::acosf(x); }
+  static __device__ __forceinline__ float cos(float x) { return ::__cosf(x); }
+  static __device__ __forceinline__ float abs(float x) { return __builtin_fabsf(x); }
+};
+
+template <typename T> __device__ __forceinline__ bool is_nan(T x) { return x != x; }
+
+// ------------------------------------------------------------------ metric
+// g = eta + f l l, l_mu = (1, (r x + a y)/(r^2+a^2), (r y - a x)/(r^2+a^2), z/r)
+template <typename T> struct KS {
+  T f, l1, l2, l3;
+};
+
+template <typename T>
+__device__ __forceinline__ KS<T> ks_metric(T x, T y, T z, T a) {
+  using M = Math<T>;
+  const T aa = a * a, zz = z * z;
+  const T kk = T(0.5) * (x * x + y * y + zz - aa);
+  const T rr = M::sqrt(kk * kk + aa * zz) + kk;
+  const T r = M::sqrt(rr);
+  KS<T> m;
+  m.f = (T(2) * rr * r) * M::rcp(rr * rr + aa * zz);
+  const T s = M::rcp(rr + aa);
+  m.l1 = (r * x + a * y) * s;
+  m.l2 = (r * y - a * x) * s;
+  m.l3 = z * M::rcp(r);
+  return m;
+}
+
+// dv/dlambda = -Gamma^alpha_{mu nu} v^mu v^nu for g = eta + f l l.
+// With L = l.v, W_i = v^s d_s l_i, D_s = d_s l_mu v^mu, F_s = d_s f:
+//   c_b = 1/2 F_b L^2 + f L D_b - l_b (L F.v + f W.v) - f L W_b
+//   a^alpha = eta^{alpha b} c_b - f l^alpha (l^b c_b),  l^mu = (-1, l_i).
+template <typename T>
+__device__ __forceinline__ void ks_accel(T x, T y, T z, T v0, T v1, T v2, T v3, T a, T acc[4]) {
+  using M = Math<T>;
+  const T aa = a * a, zz = z * z;
+  const T kk = T(0.5) * (x * x + y * y + zz - aa);
+  const T Q = M::sqrt(kk * kk + aa * zz);
+  const T iQ = M::rcp(Q);
+  const T rr = Q + kk;
+  const T r = M::sqrt(rr);
+  const T ir = M::rcp(r);
+  const T den = rr * rr + aa * zz;
+  const T iden = M::rcp(den);
+  const T f = (T(2) * rr * r) * iden;
+  const T s = M::rcp(rr + aa);
+  const T l1 = (r * x + a * y) * s;
+  const T l2 = (r * y - a * x) * s;
+  const T l3 = z * ir;
+  // d rr/dx_i and d r/dx_i
+  const T g = rr * iQ;
+  const T drr1 = x * g, drr2 = y * g, drr3 = z * (rr + aa) * iQ;
+  const T h = T(0.5) * ir;
+  const T dr1 = drr1 * h, dr2 = drr2 * h, dr3 = drr3 * h;
+  // d f = f (3 dr / r - d den / den),  d den = 2 rr d rr + 2 a^2 z e_z
+  const T c3 = T(3) * ir, e2 = T(2) * rr * iden;
+  const T df1 = f * (c3 * dr1 - e2 * drr1);
+  const T df2 = f * (c3 * dr2 - e2 * drr2);
+  const T df3 = f * (c3 * dr3 - e2 * drr3 - T(2) * aa * z * iden);
+  // d l_i / d x_j
+  const T d11 = s * (dr1 * x + r - l1 * drr1), d12 = s * (dr2 * x + a - l1 * drr2), d13 = s * (dr3 * x - l1 * drr3);
+  const T d21 = s * (dr1 * y - a - l2 * drr1), d22 = s * (dr2 * y + r - l2 * drr2), d23 = s * (dr3 * y - l2 * drr3);
+  const T d31 = -l3 * dr1 * ir, d32 = -l3 * dr2 * ir, d33 = (T(1) - l3 * dr3) * ir;
+  const T L = v0 + l1 * v1 + l2 * v2 + l3 * v3;
+  const T Fv = df1 * v1 + df2 * v2 + df3 * v3;
+  const T W1 = d11 * v1 + d12 * v2 + d13 * v3;
+  const T W2 = d21 * v1 + d22 * v2 + d23 * v3;
+  const T W3 = d31 * v1 + d32 * v2 + d33 * v3;
+  const T D1 = d11 * v1 + d21 * v2 + d31 * v3;
+  const T D2 = d12 * v1 + d22 * v2 + d32 * v3;
+  const T D3 = d13 * v1 + d23 * v2 + d33 * v3;
+  const T Wv = W1 * v1 + W2 * v2 + W3 * v3;
+  const T P = L * Fv + f * Wv;
+  const T fL = f * L, hL2 = T(0.5) * L * L;
+  const T c0 = -P;
+  const T c1 = hL2 * df1 + fL * (D1 - W1) - l1 * P;
+  const T c2 = hL2 * df2 + fL * (D2 - W2) - l2 * P;
+  const T c3b = hL2 * df3 + fL * (D3 - W3) - l3 * P;
+  const T lc = -c0 + l1 * c1 + l2 * c2 + l3 * c3b;
+  const T flc = f * lc;
+  acc[0] = -c0 + flc;
+  acc[1] = c1 - l1 * flc;
+  acc[2] = c2 - l2 * flc;
+  acc[3] = c3b - l3 * flc;
+}
+
+// RK4 over the 8-vector state, all in T (geodesics.py:317-336).
+template <typename T>
+__device__ __forceinline__ void rk4_step(const T s[8], T dt, T a, T out[8]) {
+  T k[8], acc[4], t[8], sum[8];
+  // stage 1
+  ks_accel<T>(s[1], s[2], s[3], s[4], s[5], s[6], s[7], a, acc);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) { k[i] = dt * s[4 + i]; k[4 + i] = dt * acc[i]; }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) { sum[i] = k[i]; t[i] = s[i] + T(0.5) * k[i]; }
+  // stage 2
+  ks_accel<T>(t[1], t[2], t[3], t[4], t[5], t[6], t[7], a, acc);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) { k[i] = dt * t[4 + i]; k[4 + i] = dt * acc[i]; }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) { sum[i] += T(2) * k[i]; t[i] = s[i] + T(0.5) * k[i]; }
+  // stage 3
+  ks_accel<T>(t[1], t[2], t[3], t[4], t[5], t[6], t[7], a, acc);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) { k[i] = dt * t[4 + i]; k[4 + i] = dt * acc[i]; }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) { sum[i] += T(2) * k[i]; t[i] = s[i] + k[i]; }
+  // stage 4
+  ks_accel<T>(t[1], t[2], t[3], t[4], t[5], t[6], t[7], a, acc);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) { k[i] = dt * t[4 + i]; k[4 + i] = dt * acc[i]; }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) out[i] = s[i] + T(1.0 / 6.0) * (sum[i] + k[i]);
+}
+
+// Mixed step: stages in FP32, increment added to the FP64 state.
+__device__ __forceinline__ void rk4_step_mixed(const double s[8], double dt, float a, double out[8]) {
+  float sf[8], inc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) sf[i] = (float)s[i];
+  const float dtf = (float)dt;
+  float k[8], acc[4], t[8], sum[8];
+  ks_accel<float>(sf[1], sf[2], sf[3], sf[4], sf[5], sf[6], sf[7], a, acc);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) { k[i] = dtf * sf[4 + i]; k[4 + i] = dtf * acc[i]; }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) { sum[i] = k[i]; t[i] = sf[i] + 0.5f * k[i]; }
+  ks_accel<float>(t[1], t[2], t[3], t[4], t[5], t[6], t[7], a, acc);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) { k[i] = dtf * t[4 + i]; k[4 + i] = dtf * acc[i]; }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) { sum[i] += 2.0f * k[i]; t[i] = sf[i] + 0.5f * k[i]; }
+  ks_accel<float>(t[1], t[2], t[3], t[4], t[5], t[6], t[7], a, acc);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) { k[i] = dtf * t[4 + i]; k[4 + i] = dtf * acc[i]; }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) { sum[i] += 2.0f * k[i]; t[i] = sf[i] + k[i]; }
+  ks_accel<float>(t[1], t[2], t[3], t[4], t[5], t[6], t[7], a, acc);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) { k[i] = dtf * t[4 + i]; k[4 + i] = dtf * acc[i]; }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) inc[i] = (1.0f / 6.0f) * (sum[i] + k[i]);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) out[i] = s[i] + (double)inc[i];
+}
+
+// geodesics.py:284-291
+__device__ __forceinline__ double radius_cal(double x, double y, double z, double a) {
+#pragma clang fp contract(off)
+  const double R = __builtin_sqrt(x * x + y * y + z * z);
+  const double R2 = R * R, aa = a * a;
+  return __builtin_sqrt((R2 - aa + __builtin_sqrt((R2 - aa) * (R2 - aa) + 4.0 * aa * (z * z))) / 2.0);
+}
+
+// geodesics.py:249-252: dt = -(r - r_EH)/div, zeroed on NaN, |dt| div < tol
+// or |dt| div > 1500
+__device__ __forceinline__ double step_rule(double r, double rEH, double div, double tol) {
+#pragma clang fp contract(off)
+  const double dt = -(r - rEH) / div;
+  const double m = __builtin_fabs(dt) * div;
+  if (dt != dt || m < tol || m > 1500.0) return 0.0;
+  return dt;
+}
+
+// ------------------------------------------------------------------ camera
+// _Image_to_BH / _perpendicular / cross product (geodesics.py:204-216,
+// 137-181, 107-134), evaluated without contraction so that each product and
+// sum rounds as the NumPy expressions do.
+__device__ __forceinline__ void camera_ray(double x, double y, double ci, double si, double d, double sx[4],
+                                           double sv[4]) {
+#pragma clang fp contract(off)
+  const double ox = 0.0 * ci + 0.0 * si + d * si, oy = 0.0, oz = 0.0 * si + 0.0 * ci + d * ci;
+  const double ix = -y * ci + 0.0 * si + d * si, iy = x, iz = y * si + 0.0 * ci + d * ci;
+  const double px_ = x + y, py_ = y - x;
+  const double qx = -py_ * ci + 0.0 * si + d * si, qy = px_, qz = py_ * si + 0.0 * ci + d * ci;
+  const double a0 = -ix + ox, a1 = -iy + oy, a2 = -iz + oz;
+  const double b0 = qx - ix, b1 = qy - iy, b2 = qz - iz;
+  sx[0] = 0.0; sx[1] = ix; sx[2] = iy; sx[3] = iz;
+  sv[0] = 1.0;
+  sv[1] = a1 * b2 - a2 * b1;
+  sv[2] = a2 * b0 - a0 * b2;
+  sv[3] = a0 * b1 - a1 * b0;
+}
+
+// numpy.linspace(lo, hi, 2n+1)[1::2]
+__device__ __forceinline__ double grid_coord(double lo, double step, int i) {
+#pragma clang fp contract(off)
+  return (double)(2 * i + 1) * step + lo;
+}
+
+// _quadratic + nullify (geodesics.py:58-85) with k^t = v0 held fixed
+__device__ __forceinline__ void nullify(const double x[4], const double v[4], double a, double out[8]) {
+#pragma clang fp contract(off)
+  const KS<double> m = ks_metric<double>(x[1], x[2], x[3], a);
+  const double l[4] = {1.0, m.l1, m.l2, m.l3};
+  double g[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) g[i][j] = (i == j ? (i == 0 ? -1.0 : 1.0) : 0.0) + m.f * (l[i] * l[j]);
+  const double A = v[0] * g[0][0] * v[0];
+  const double b = (v[1] * g[1][0] + v[2] * g[2][0] + v[3] * g[3][0]) * v[0];
+  double C = 0.0;
+#pragma unroll
+  for (int j = 1; j < 4; ++j) {
+    const double t = v[1] * g[1][j] + v[2] * g[2][j] + v[3] * g[3][j];
+    C += t * v[j];
+  }
+  const double bb = b * b, AC = A * C;
+  const bool close = __builtin_fabs(bb - AC) <= 1e-8 + 1e-5 * __builtin_fabs(AC);
+  const double dd = close ? 0.0 : bb - AC;
+  const double bs = (b < 0.0) ? 0.0 : 1.0;  // heaviside(b, 1)
+  const double D = -(b + bs * __builtin_sqrt(dd));
+  const double x1 = D / A, x2 = C / D;
+  const bool nan = (x1 != x1) || (x2 != x2);
+  const double lo = nan ? __builtin_nan("") : (x1 < x2 ? x1 : x2);
+  const double hi = nan ? __builtin_nan("") : (x1 > x2 ? x1 : x2);
+  const double S = lo > 0.0 ? lo : (hi > 0.0 ? hi : __builtin_nan(""));
+  out[0] = x[0]; out[1] = x[1]; out[2] = x[2]; out[3] = x[3];
+  out[4] = v[0];
+  out[5] = v[1] / S; out[6] = v[2] / S; out[7] = v[3] / S;
+}
+
+// ------------------------------------------------------------------ snapshot
+// Device view of a ghost-padded meshblock snapshot.  Cells are stored as
+// records of the 8 primitives in canonical order [rho, u, U1, U2, U3, B1,
+// B2, B3] (32 bytes, two float4), block-major then k, j, i, so the 2 x-
+// neighbours of a trilinear stencil are 64 contiguous bytes.
+struct SnapView {
+  const float4* cells;   // nmb * n2^3 * 2
+  const double* meta;    // nmb * 16: lo[3], hi[3], v0[3], dx[3], idx[3], pad
+  const int* grid;       // gz * gy * gx leaf ids (-1 = hole)
+  int nmb, n2;
+  int gx, gy, gz;
+  double g0[3], ginv[3], dom_lo[3], dom_hi[3];
+};
+
+__device__ __forceinline__ bool in_block(const double* mb, double x, double y, double z) {
+  return (mb[0] < x && x <= mb[3]) && (mb[1] < y && y <= mb[4]) && (mb[2] < z && z <= mb[5]);
+}
+
+// Leaf id with the reference's rule (left-open, right-closed, last match
+// wins; athenak.py:657-670), via the level-resolution lookup grid.
+__device__ __forceinline__ int find_block(const SnapView& sv, double x, double y, double z) {
+  if (!(sv.dom_lo[0] < x && x <= sv.dom_hi[0] && sv.dom_lo[1] < y && y <= sv.dom_hi[1] && sv.dom_lo[2] < z &&
+        z <= sv.dom_hi[2]))
+    return -1;
+  int c[3];
+  const double p[3] = {x, y, z};
+  const int gn[3] = {sv.gx, sv.gy, sv.gz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int ci = (int)__builtin_ceil((p[d] - sv.g0[d]) * sv.ginv[d]) - 1;
+    ci = ci < 0 ? 0 : (ci >= gn[d] ? gn[d] - 1 : ci);
+    c[d] = ci;
+  }
+  // the cell and, for points within rounding of a face, its neighbours
+  for (int dz = 0; dz <= 2; ++dz)
+    for (int dy = 0; dy <= 2; ++dy)
+      for (int dx = 0; dx <= 2; ++dx) {
+        const int cx = c[0] + (dx == 2 ? -1 : dx), cy = c[1] + (dy == 2 ? -1 : dy), cz = c[2] + (dz == 2 ? -1 : dz);
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= sv.gx || cy >= sv.gy || cz >= sv.gz) continue;
+        const int b = sv.grid[((long)cz * sv.gy + cy) * sv.gx + cx];
+        if (b >= 0 && in_block(sv.meta + 16 * (long)b, x, y, z)) return b;
+      }
+  int found = -1;  // holes in the tiling: scan, as the reference does
+  for (int b = 0; b < sv.nmb; ++b)
+    if (in_block(sv.meta + 16 * (long)b, x, y, z)) found = b;
+  return found;
+}
+
+template <typename T>
+__device__ __forceinline__ T lerp_ref(T a, T b, T t) {
+  return a + (b - a) * t;
+}
+
+// Trilinear primitives at (x, y, z) in block mb (athenak.py:718-757).
+template <typename T>
+__device__ __forceinline__ void trilinear(const SnapView& sv, int mb, double x, double y, double z, T P[8]) {
+  if (mb < 0) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) P[k] = T(0);
+    return;
+  }
+  const double* m = sv.meta + 16 * (long)mb;
+  const double p[3] = {x, y, z};
+  int id[3];
+  T fr[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double q = p[d] - m[6 + d] + m[9 + d];
+    const double t = q / m[9 + d];
+    const double fl = __builtin_floor(t);
+    int i = (int)fl;
+    i = i < 0 ? 0 : (i > sv.n2 - 2 ? sv.n2 - 2 : i);
+    id[d] = i;
+    fr[d] = (T)(t - fl);
+  }
+  const long n2 = sv.n2;
+  const float4* c = sv.cells + 2 * (((long)mb * n2 + id[2]) * n2 * n2 + (long)id[1] * n2 + id[0]);
+  const long sj = 2 * n2, sk = 2 * n2 * n2;
+  float4 q[8][2];
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 2; ++ii) {
+        const float4* e = c + kk * sk + jj * sj + 2 * ii;
+        q[kk * 4 + jj * 2 + ii][0] = e[0];
+        q[kk * 4 + jj * 2 + ii][1] = e[1];
+      }
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    T s[8];
+#pragma unroll
+    for (int c8 = 0; c8 < 8; ++c8) {
+      const float4 w = q[c8][v >> 2];
+      const int r = v & 3;
+      s[c8] = (T)(r == 0 ? w.x : (r == 1 ? w.y : (r == 2 ? w.z : w.w)));
+    }
+    // naming of the reference: d{x3}{x2}{x1}, index = x3*4 + x2*2 + x1
+    const T daa = lerp_ref(s[0], s[1], fr[0]);
+    const T dab = lerp_ref(s[2], s[3], fr[0]);
+    const T dba = lerp_ref(s[4], s[5], fr[0]);
+    const T dbb = lerp_ref(s[6], s[7], fr[0]);
+    const T da = lerp_ref(daa, dab, fr[1]);
+    const T db = lerp_ref(dba, dbb, fr[1]);
+    P[v] = lerp_ref(da, db, fr[2]);
+  }
+}
+
+// Fluid scalars at one state (athenak.py:759-794): returns rho, u, pitch,
+// k.u, |b|.  Uses g^{t mu} = eta^{t mu} - f l^t l^mu in closed form.
+template <typename T>
+__device__ __forceinline__ void fluid_scalars(const T P[8], T x, T y, T z, T k0, T k1, T k2, T k3, T a,
+                                              T fallback_cos, T out[5]) {
+  using M = Math<T>;
+  const KS<T> m = ks_metric<T>(x, y, z, a);
+  const T f = m.f, l1 = m.l1, l2 = m.l2, l3 = m.l3;
+  const T alpha = M::sqrt(T(1) / (T(1) + f));
+  const T U1 = P[2], U2 = P[3], U3 = P[4];
+  const T lU = l1 * U1 + l2 * U2 + l3 * U3;
+  const T gamma = M::sqrt(T(1) + (U1 * U1 + U2 * U2 + U3 * U3 + f * lU * lU));
+  const T ga = gamma * alpha;
+  const T uc0 = gamma / alpha;
+  const T uc1 = U1 - ga * (f * l1), uc2 = U2 - ga * (f * l2), uc3 = U3 - ga * (f * l3);
+  const T Lu = uc0 + l1 * uc1 + l2 * uc2 + l3 * uc3;
+  const T fLu = f * Lu;
+  const T ul0 = -uc0 + fLu, ul1 = uc1 + l1 * fLu, ul2 = uc2 + l2 * fLu, ul3 = uc3 + l3 * fLu;
+  const T B1 = P[5], B2 = P[6], B3 = P[7];
+  const T bc0 = B1 * ul1 + B2 * ul2 + B3 * ul3;
+  const T iu0 = T(1) / uc0;
+  const T bc1 = (B1 + uc1 * bc0) * iu0, bc2 = (B2 + uc2 * bc0) * iu0, bc3 = (B3 + uc3 * bc0) * iu0;
+  const T Lb = bc0 + l1 * bc1 + l2 * bc2 + l3 * bc3;
+  const T fLb = f * Lb;
+  const T bl0 = -bc0 + fLb, bl1 = bc1 + l1 * fLb, bl2 = bc2 + l2 * fLb, bl3 = bc3 + l3 * fLb;
+  const T kdotu = k0 * ul0 + k1 * ul1 + k2 * ul2 + k3 * ul3;
+  const T kdotb = k0 * bl0 + k1 * bl1 + k2 * bl2 + k3 * bl3;
+  const T bdotb = bc0 * bl0 + bc1 * bl1 + bc2 * bl2 + bc3 * bl3;
+  T p = kdotb / (M::abs(kdotu) * M::sqrt(bdotb));
+  if (is_nan(p)) p = fallback_cos;
+  if (M::abs(p) > T(1)) p = p / M::abs(p);
+  out[0] = P[0];
+  out[1] = P[1];
+  out[2] = M::acos(p);
+  out[3] = kdotu;
+  out[4] = M::sqrt(bdotb);
+}
+
+// ------------------------------------------------------------- emission
+// transfer.py:30-86 for one sample
+template <typename T>
+__device__ __forceinline__ void synchrotron(T Ne, T Th, T B, T pitch, T nu, bool invariant, T rescale, T& jo,
+                                            T& ao) {
+  using M = Math<T>;
+  const T nuc = T(kEE) * B / T(2.0 * kPI * kME * kCL);
+  const T nus = T(2.0 / 9.0) * nuc * (Th * Th) * M::sin(pitch);
+  const T X = nu / nus;
+  const T var = M::exp(-M::pow(X, T(1.0 / 3)));
+  const T term = M::sqrt(X) + T(1.8877486253633870) * M::pow(X, T(1.0 / 6));  // 2^(11/12)
+  T em = Ne * nus * (term * term) / (T(2) * (Th * Th));
+  em = em * var * T(1.4142135623730951 * kPI * kEE * kEE / (3.0 * kCL));
+  if (X > T(1e12)) em = T(0);
+  if (Th < T(0.3)) em = T(0);
+  const T bx = T(kHPL) * nu / (T(kME * kCL * kCL) * Th);
+  const T series = bx / T(24) * (T(24) + bx * (T(12) + bx * (T(4) + bx)));
+  const T Bden = (bx < T(2e-3)) ? series : (M::exp(bx) - T(1));
+  const T Bnu = (T(2.0 * kHPL) * (nu * nu * nu) / Bden) / T(kCL * kCL);
+  T ab = em / Bnu;
+  if (invariant) {
+    const T rn = nu * rescale;
+    em = em / (rn * rn);
+    ab = ab * rn;
+  }
+  if (is_nan(em)) em = T(0);
+  if (is_nan(ab)) ab = T(0);
+  jo = em;
+  ao = ab;
+}
+
+// electrons.py:32-50
+template <typename T>
+__device__ __forceinline__ T rlow_rhigh(T dens, T u, T beta, T r_low, T r_high, T ge, T gi) {
+  const T b2 = beta * beta;
+  const T Tr = (r_high * b2 + r_low) / (T(1) + b2);
+  T te = T(kCL * kCL) * (T(kMP) * u * (ge - T(1)) * (gi - T(1)));
+  te = te / (dens * ((gi - T(1)) + (ge - T(1)) * Tr));
+  return te / T(kME * kCL * kCL);
+}
+
+// Per-sample image parameters (images.py:86-118, grmhd.py:40-55)
+struct ImageParams {
+  double fluid_gamma, r_high, Ne_unit, B_unit, L_unit, nu_obs, fallback_cos, sigma_cut;
+};
+
+// (j, alpha) at one accepted state (images.py:86-118)
+template <typename T>
+__device__ __forceinline__ void emission(const SnapView& sv, const ImageParams& ip, const double s[8], double a,
+                                         T& j, T& al) {
+  const int mb = find_block(sv, s[1], s[2], s[3]);
+  T P[8];
+  trilinear<T>(sv, mb, s[1], s[2], s[3], P);
+  T sc[5];
+  fluid_scalars<T>(P, (T)s[1], (T)s[2], (T)s[3], (T)s[4], (T)s[5], (T)s[6], (T)s[7], (T)a, (T)ip.fallback_cos, sc);
+  const T dens = sc[0], u = sc[1], pitch = sc[2], kdotu = sc[3], b = sc[4];
+  const T bsq = b * b;
+  const T beta = u * (T(ip.fluid_gamma) - T(1)) / bsq / T(0.5);
+  const T sigma = bsq / dens;
+  const T Th = rlow_rhigh<T>(dens, u, beta, T(1), T(ip.r_high), T(4.0 / 3), T(5.0 / 3));
+  const T Ne = T(ip.Ne_unit) * dens, Bg = T(ip.B_unit) * b, nu = -kdotu * T(ip.nu_obs);
+  synchrotron<T>(Ne, Th, Bg, pitch, nu, true, T(1.0 / ip.nu_obs), j, al);
+  if (sigma > T(ip.sigma_cut)) {
+    j = T(0);
+    al = T(0);
+  }
+}
+
+}  // namespace mk

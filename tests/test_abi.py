@@ -1,0 +1,72 @@
+"""CPU checks of the C ABI and host logic (no GPU needed)."""
+import ctypes as C
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from mahakala_amd import _lib
+from mahakala_amd._array import MkArray, wrap
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = _lib.lib()
+    declared = _lib.declared_symbols()
+    assert len(declared) >= 40
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (mk_\w+)", out))
+    missing = [s for s in declared if s not in exported]
+    assert not missing, missing
+    for s in declared:
+        assert hasattr(L, s)
+    assert L.mk_version() >= 10000
+
+
+def test_argument_errors_are_reported_without_a_device():
+    L = _lib.lib()
+    rc = L.mk_camera_grid(0.5, 30.0, 1000.0, -10.0, 10.0, 0, None, None)
+    assert rc == -1
+    assert b"mk_camera_grid" in L.mk_last_error()
+    p = _lib.mk_render_params()
+    assert L.mk_render(None, C.byref(p), None, None, None) == -1
+    with pytest.raises(_lib.MkError):
+        _lib.check(L.mk_integrate_history(None, 1, 10, 40.0, 1e-4, 0.5, 5, None, None, None))
+
+
+def test_render_params_layout_matches_header():
+    text = open(_lib.HEADER_PATH).read()
+    block = text[text.index("typedef struct mk_render_params"):text.index("} mk_render_params;")]
+    fields = re.findall(r"^\s*(double|int32_t)\s+(\w+);", block, re.M)
+    assert [n for _, n in fields] == [n for n, _ in _lib.mk_render_params._fields_]
+    assert C.sizeof(_lib.mk_render_params) == 16 * 8 + 4 * 4
+
+
+def test_mkarray_at_is_functional():
+    a = wrap(np.arange(6.0))
+    b = a.at[a > 2].set(0.0)
+    assert isinstance(b, MkArray)
+    assert np.array_equal(a, np.arange(6.0))
+    assert np.array_equal(b, [0, 1, 2, 0, 0, 0])
+    c = b.at[np.array([0, 1])].add(10.0)
+    assert np.array_equal(c, [10, 11, 2, 0, 0, 0])
+    assert np.array_equal(a.at[1:3].get(), [1, 2])
+    assert np.array_equal(a.at[a > 4].multiply(2.0), [0, 1, 2, 3, 4, 10])
+
+
+def test_mahakala_alias_exposes_reference_surface():
+    import mahakala as ma
+    import mahakala.geodesics as mg
+    import mahakala_amd.geodesics as ag
+    from mahakala.grmhd.athenak import AthenakFluidModel  # noqa: F401
+    from mahakala.images import make_image  # noqa: F401
+    from mahakala.electrons import rlow_rhigh_model  # noqa: F401
+    from mahakala.constants import Msun, EE, KB, CL, ME, HPL, GNEWT  # noqa: F401
+    assert mg is ag
+    assert set(ma.__all__) == {"find_shadow_bisection", "find_shadow_bisection_angles", "geodesic_integrator",
+                               "initialize_geodesics_at_camera", "synchrotron_coefficients",
+                               "solve_specific_intensity", "solve_attenuated_emissivity"}
+    for name in ("metric", "imetric", "radius_cal", "radius_EH", "RK4_gen", "rhs", "initial_condition",
+                 "get_initial_grid", "get_camera_pixel", "select_photons_integrator"):
+        assert callable(getattr(mg, name))

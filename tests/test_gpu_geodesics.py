@@ -1,0 +1,120 @@
+"""GPU parity of the geodesic kernels against the FP64 oracles.
+
+Reference functions: geodesics.py:29-435.  Tolerances: camera and metric
+quantities to ~1e-13 relative (rounding of a different but equivalent
+expression), trajectories to 1e-8 relative over ~900 RK4 steps, integer
+outcomes (frozen row K, trimmed length T, bisection radii) exact.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import mahakala as ma
+from mahakala_amd import geodesics as G
+from oracle import c_oracle
+from oracle import mahakala_np as onp
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.mark.parametrize("a,inc,n", [(0.9, 60.0, 8), (0.5, 30.0, 7), (-0.7, 85.0, 16), (0.0, 0.0, 5)])
+def test_camera_grid(a, inc, n):
+    gpu = np.asarray(ma.initialize_geodesics_at_camera(a, inc, 1000, -10, 10, n))
+    ref = onp.initialize_geodesics_at_camera(a, inc, 1000, -10, 10, n)
+    np.testing.assert_allclose(gpu, ref, rtol=1e-13, atol=1e-12, equal_nan=True)
+    if n % 2:  # the centre pixel has k = 0 and is NaN in the reference too
+        assert np.isnan(gpu[(n * n) // 2, 5:]).all()
+
+
+def test_camera_equator_and_pixels():
+    gpu = np.asarray(ma.initialize_geodesics_at_camera(0.0, 60, 1000, -15, 15, 60, camera_type="Equator"))
+    ref = onp.initialize_geodesics_at_camera(0.0, 60, 1000, -15, 15, 60, camera_type="equator")
+    np.testing.assert_allclose(gpu, ref, rtol=1e-13, atol=1e-13)
+    rad = np.linspace(0.5, 10, 9)
+    ang = np.linspace(0, 6, 9)
+    sx, sv = G.get_camera_pixel(40, 1000, rad, ang)
+    gpu = np.asarray(G.initial_condition(sx, sv, 0.3))
+    rx, rv = onp.get_camera_pixel(40, 1000, rad, ang)
+    np.testing.assert_allclose(gpu, onp.initial_condition(rx, rv, 0.3), rtol=1e-13, atol=1e-12)
+
+
+def test_metric_rhs_rk4():
+    rng = np.random.Generator(np.random.PCG64(5))
+    x = np.zeros((200, 4))
+    x[:, 1:] = rng.uniform(-30, 30, (200, 3))
+    for a in (0.0, 0.6, -0.95):
+        np.testing.assert_allclose(G.metric(x, a), onp.metric(x, a), rtol=1e-13, atol=1e-14)
+        np.testing.assert_allclose(G.imetric(x, a), onp.imetric(x, a), rtol=1e-12, atol=1e-13)
+        s = np.concatenate([x, rng.normal(size=(200, 4))], 1)
+        ref = onp.rhs(s, a)
+        np.testing.assert_allclose(G.rhs(s, a), ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+        dt = rng.uniform(-1, 0, 200)
+        np.testing.assert_allclose(G.RK4_gen(s, dt, a), onp.RK4_gen(s, dt, a), rtol=1e-11, atol=1e-11)
+        np.testing.assert_allclose(G.radius_cal(x, a), onp.radius_cal(x, a), rtol=1e-15)
+
+
+def test_integrator_history_vs_oracle():
+    f = np.load(os.path.join(GOLD, "geodesic_6x6.npz"))
+    a = float(f["spin"])
+    S, dt = G.geodesic_integrator(10000, f["s0"], 40, 1e-4, a)
+    assert S.shape == (int(f["T"]), 36, 8)
+    np.testing.assert_allclose(np.asarray(S)[f["rows"]], f["S_rows"], rtol=1e-8, atol=1e-8)
+    np.testing.assert_allclose(np.asarray(dt)[f["rows"]], f["dt_rows"], rtol=1e-8, atol=1e-10)
+    # frozen row per ray = first zero of dt
+    K = np.where((np.asarray(dt) == 0).any(0), np.argmax(np.asarray(dt) == 0, 0), 10000)
+    np.testing.assert_array_equal(K, f["K"])
+
+
+def test_integrate_final_config1():
+    """64x64, a=0.9, i=60, camera at 1000 M; tol = 0.01 r_EH (the '1.01 r_h' stop)."""
+    a = 0.9
+    rEH = 1 + np.sqrt(1 - a * a)
+    s0 = np.asarray(ma.initialize_geodesics_at_camera(a, 60, 1000, -10, 10, 64))
+    import ctypes as C
+    from mahakala_amd import _lib
+    n = s0.shape[0]
+    d0 = _lib.DeviceBuffer.from_array(s0)
+    dK, dl, ds = _lib.DeviceBuffer(4 * n), _lib.DeviceBuffer(64 * n), _lib.DeviceBuffer(8 * n)
+    _lib.check(_lib.lib().mk_integrate_final(d0.ptr, n, 10000, 40.0, 0.01 * rEH, a, dK.ptr, dl.ptr, ds.ptr, None))
+    K = dK.to_array((n,), np.int32)
+    last = dl.to_array((n, 8))
+    Kc, lastc, _ = c_oracle.integrate_final(s0, 10000, 40.0, 0.01 * rEH, a)
+    assert np.mean(K == Kc) >= 0.999
+    same = K == Kc
+    r_g = onp.radius_cal(last[same], a)
+    r_c = onp.radius_cal(lastc[same], a)
+    np.testing.assert_allclose(r_g, r_c, rtol=1e-6)
+    T = C.c_long()
+    _lib.check(_lib.lib().mk_trim_rows(dK.ptr, n, 10000, C.byref(T), None))
+    assert T.value == c_oracle.trim_rows(Kc, 10000)
+
+
+def test_shadow_reference_cases():
+    """tests/test_shadows.py of the reference, on the closed-form fixture, plus
+    identity with the C restatement (the result is dyadic)."""
+    f = np.load(os.path.join(GOLD, "shadow_bardeen.npz"))
+    for c in range(4):
+        a, inc = float(f[f"case{c}_spin"]), float(f[f"case{c}_inc"])
+        ang, target = f[f"case{c}_angles"], f[f"case{c}_radii"]
+        radii = ma.find_shadow_bisection_angles(a, inc, ang)
+        assert np.allclose(radii, target, rtol=1e-2)
+        np.testing.assert_array_equal(radii, c_oracle.find_shadow_bisection_angles(a, inc, ang))
+
+
+def test_shadow_config1_and_select():
+    ang, radii = ma.find_shadow_bisection(0.9, 60, 64)
+    assert ang.shape == (65,) and radii[0] == radii[-1]
+    ref = c_oracle.find_shadow_bisection_angles(0.9, 60, ang[:-1])
+    assert np.max(np.abs(radii[:-1] - ref)) <= 1e-6
+    f = np.load(os.path.join(GOLD, "shadow_oracle.npz"))
+    np.testing.assert_array_equal(ma.find_shadow_bisection_angles(float(f["spin"]), float(f["inc"]), f["angles"]),
+                                  f["radii"])
+    sel = G.select_photons_integrator(float(f["inc"]), f["angles"], np.full(6, 5.0), float(f["spin"]))
+    np.testing.assert_allclose(sel, f["select_r"], rtol=1e-8)
+
+
+def test_schwarzschild_shadow():
+    r = ma.find_shadow_bisection_angles(0.0, 17.0, np.linspace(0, 6, 12))
+    assert np.all(np.abs(r - np.sqrt(27)) < 9.5 / 2 ** 14 + 1e-3)
