@@ -78,8 +78,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--res", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--only", choices=["mixed", "plain64"], default=None,
+    ap.add_argument("--only", choices=["mixed", "plain64", "mixed-zone"], default=None,
                     help="time one precision only (profiling runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the device copy kernel over MIB MiB 3 times (traffic counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported (run with --gpus 1)",
@@ -88,6 +90,15 @@ def main():
 
     from mahakala_amd import _lib as L
     from mahakala_amd import images
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        src, dst = L.DeviceBuffer(nb), L.DeviceBuffer(nb)
+        L.check(L.lib().mk_memset(src.ptr, 1, nb, None))
+        for _ in range(3):
+            L.check(L.lib().mk_device_copy(dst.ptr, src.ptr, nb, None))
+        L.sync()
+        print(json.dumps({"copy_calibration_bytes": nb, "launches": 3}))
+        return
     from mahakala_amd.grmhd.athenak import AthenakFluidModel
     from mahakala_amd.grmhd.synthetic import config2_snapshot
 
@@ -101,8 +112,9 @@ def main():
     n = res * res
     img = L.DeviceBuffer(8 * n)
     stp = L.DeviceBuffer(4 * n)
+    precisions = ("plain64", "mixed", "mixed-zone")
     par = {p: images.render_params(a, camera_inclination=inc, fov=fov, resolution=res, observing_frequency=nu,
-                                   precision=p) for p in ("plain64", "mixed")}
+                                   precision=p) for p in precisions}
 
     def launch(p, steps_out=False):
         def fn():
@@ -114,13 +126,16 @@ def main():
     L.sync()
     I_plain = img.to_array((res, res))
     steps_plain = stp.to_array((res, res), np.int32)
-    launch("mixed")()
-    L.sync()
-    I_mixed = img.to_array((res, res))
+    I_mix = {}
+    for p in ("mixed", "mixed-zone"):
+        launch(p)()
+        L.sync()
+        I_mix[p] = img.to_array((res, res))
+    I_mixed = I_mix["mixed"]
     total_steps = float(steps_plain.sum())
 
     results = {}
-    for p in ("plain64", "mixed"):
+    for p in precisions:
         if args.only and p != args.only:
             continue
         ms, wall = time_launches(L, launch(p), args.steps, args.warmup)
@@ -128,10 +143,15 @@ def main():
         print(f"[bench] {p}: {np.mean(ms):.3f} ms/image (p50 {np.percentile(ms, 50):.3f}, "
               f"p99 {np.percentile(ms, 99):.3f}), wall {wall:.3f}s for {args.steps}", file=sys.stderr, flush=True)
 
-    flux_p, flux_m = float(I_plain.sum()), float(I_mixed.sum())
-    acc = dict(flux_rel_err=abs(flux_m - flux_p) / flux_p,
-               pixel_l1_rel=float(np.abs(I_mixed - I_plain).sum() / I_plain.sum()),
-               ring_diameter_plain=ring_diameter(I_plain, fov), ring_diameter_mixed=ring_diameter(I_mixed, fov))
+    flux_p = float(I_plain.sum())
+
+    def accuracy(I):
+        bright = I_plain > 1e-3 * I_plain.max()
+        return dict(flux_rel_err=abs(float(I.sum()) - flux_p) / flux_p,
+                    pixel_l1_rel=float(np.abs(I - I_plain).sum() / I_plain.sum()),
+                    pixel_max_rel_bright=float((np.abs(I - I_plain)[bright] / I_plain[bright]).max()),
+                    ring_diameter_plain=ring_diameter(I_plain, fov), ring_diameter_mixed=ring_diameter(I, fov))
+    acc = accuracy(I_mixed)
 
     head = args.only or "mixed"
     ms_head = results[head]["ms"]
@@ -174,6 +194,14 @@ def main():
                              "achieved_tflops": round(a64, 3), "peak": PEAK_FP64, "frac": round(a64 / PEAK_FP64, 4)}
         if "mixed" in results:
             out["speedup_mixed_over_plain"] = round(pw / results["mixed"]["wall"], 3)
+        if "mixed-zone" in results:
+            zw = results["mixed-zone"]["wall"]
+            out["mixed_zone"] = {"what": "FP64 RK4 substeps inside r < 4 M and near the photon ring",
+                                 "value": round(n * args.steps / zw, 1), "unit": "rays/s",
+                                 "ms_per_image": round(float(np.mean(results["mixed-zone"]["ms"])), 4),
+                                 "speedup_over_plain": round(pw / zw, 3),
+                                 "accuracy_vs_plain": {k: (round(v, 9) if isinstance(v, float) else v)
+                                                       for k, v in accuracy(I_mix["mixed-zone"]).items()}}
 
     if not args.no_cpu:
         from oracle import c_oracle

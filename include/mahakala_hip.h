@@ -46,6 +46,9 @@ int mk_free(void* d_ptr);
 int mk_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes, void* stream);
 int mk_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes, void* stream);
 int mk_memset(void* d_dst, int value, size_t bytes, void* stream);
+/* device-to-device copy by a shader kernel (16-byte granules); no reference
+ * counterpart: used to calibrate HBM traffic counters against known bytes */
+int mk_device_copy(void* d_dst, const void* d_src, size_t bytes, void* stream);
 int mk_stream_create(void** stream);
 int mk_stream_destroy(void* stream);
 int mk_stream_sync(void* stream);

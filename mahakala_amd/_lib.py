@@ -47,6 +47,7 @@ _SIGS = {
     "mk_memcpy_h2d": [vp, vp, C.c_size_t, vp],
     "mk_memcpy_d2h": [vp, vp, C.c_size_t, vp],
     "mk_memset": [vp, C.c_int, C.c_size_t, vp],
+    "mk_device_copy": [vp, vp, C.c_size_t, vp],
     "mk_stream_create": [C.POINTER(vp)],
     "mk_stream_destroy": [vp],
     "mk_stream_sync": [vp],

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -280,6 +281,11 @@ __global__ void k_rlow_rhigh(const double* dens, const double* u, const double* 
   out[p] = rlow_rhigh<double>(dens[p], u[p], beta[p], rl, rh, ge, gi);
 }
 
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 double horizon(double a) { return 1.0 + std::sqrt(1.0 - a * a); }
 
 }  // namespace mk
@@ -336,6 +342,15 @@ int mk_memcpy_d2h(void* h, const void* d, size_t bytes, void* s) {
 int mk_memset(void* d, int v, size_t bytes, void* s) {
   if (!bytes) return MK_OK;
   MK_HIP(hipMemsetAsync(d, v, bytes, as_stream(s)));
+  return MK_OK;
+}
+int mk_device_copy(void* d, const void* src, size_t bytes, void* s) {
+  MK_REQUIRE(d && src && bytes % 16 == 0, "mk_device_copy: pointers must be non-null and bytes a multiple of 16");
+  if (!bytes) return MK_OK;
+  const size_t n = bytes / 16;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+  k_copy16<<<blocks, 256, 0, as_stream(s)>>>((const uint4*)src, (uint4*)d, n);
+  MK_LAUNCH_CHECK();
   return MK_OK;
 }
 int mk_stream_create(void** s) {

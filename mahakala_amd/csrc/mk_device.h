@@ -27,14 +27,38 @@ constexpr double kHPL = 6.6261e-27;
 constexpr double kPI = 3.141592653589793;
 
 // ---------------------------------------------------------------- precision
-// Plain FP64 uses IEEE division and square root.  The FP32 path uses the
-// hardware reciprocal / square root (about 1 ulp) and exp2/log2.
+// FP64: division and square root on the geodesic right-hand side use the
+// hardware reciprocal / reciprocal square root refined by Newton steps
+// (within 1 ulp of IEEE; positive normal arguments only there); the step
+// rule, radius and camera keep IEEE operations.  FP32: hardware reciprocal,
+// square root, exp2 and log2 (about 1 ulp).
 template <typename T> struct Math;
 template <> struct Math<double> {
-  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
-  static __device__ __forceinline__ double sqrt(double x) { return __builtin_sqrt(x); }
+  static __device__ __forceinline__ double rcp(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-x, y, 1.0);
+    return __builtin_fma(y, e, y);
+  }
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+  static __device__ __forceinline__ double sqrt(double x) {
+    // Goldschmidt refinement of rsq, then two residual corrections
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return x == 0.0 ? x : g;
+  }
+  static __device__ __forceinline__ double sqrt_ieee(double x) { return __builtin_sqrt(x); }
   static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
   static __device__ __forceinline__ double pow(double x, double y) { return ::pow(x, y); }
+  static __device__ __forceinline__ double cbrt(double x) { return ::cbrt(x); }
   static __device__ __forceinline__ double sin(double x) { return ::sin(x); }
   static __device__ __forceinline__ double acos(double x) { return ::acos(x); }
   static __device__ __forceinline__ double cos(double x) { return ::cos(x); }
@@ -42,11 +66,16 @@ template <> struct Math<double> {
 };
 template <> struct Math<float> {
   static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float sqrt_ieee(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
   static __device__ __forceinline__ float pow(float x, float y) {
     // x >= 0 here; log2(0) = -inf gives 0, log2(inf) = inf gives inf
     return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+  }
+  static __device__ __forceinline__ float cbrt(float x) {
+    return __builtin_amdgcn_exp2f(0.3333333333333333f * __builtin_amdgcn_logf(x));
   }
   static __device__ __forceinline__ float sin(float x) { return ::__sinf(x); }
   static __device__ __forceinline__ float acos(float x) { return ::acosf(x); }
@@ -279,9 +308,13 @@ __device__ __forceinline__ void nullify(const double x[4], const double v[4], do
 // records of the 8 primitives in canonical order [rho, u, U1, U2, U3, B1,
 // B2, B3] (32 bytes, two float4), block-major then k, j, i, so the 2 x-
 // neighbours of a trilinear stencil are 64 contiguous bytes.
+// Per-block geometry (kMeta doubles): lo[3] = x{1,2,3}f[0], hi[3] =
+// x{1,2,3}f[-1], q0[3] = x{1,2,3}v[0] - dx, inv[3] = 1 / dx, dx = v[1] - v[0].
+constexpr int kMeta = 12;
+
 struct SnapView {
   const float4* cells;   // nmb * n2^3 * 2
-  const double* meta;    // nmb * 16: lo[3], hi[3], v0[3], dx[3], idx[3], pad
+  const double* meta;    // nmb * kMeta
   const int* grid;       // gz * gy * gx leaf ids (-1 = hole)
   int nmb, n2;
   int gx, gy, gz;
@@ -293,8 +326,10 @@ __device__ __forceinline__ bool in_block(const double* mb, double x, double y, d
 }
 
 // Leaf id with the reference's rule (left-open, right-closed, last match
-// wins; athenak.py:657-670), via the level-resolution lookup grid.
-__device__ __forceinline__ int find_block(const SnapView& sv, double x, double y, double z) {
+// wins; athenak.py:657-670), via the level-resolution lookup grid.  `meta`
+// and `grid` are the snapshot's tables, in global memory or copied to LDS.
+__device__ __forceinline__ int find_block(const SnapView& sv, const double* meta, const int* grid, double x,
+                                          double y, double z) {
   if (!(sv.dom_lo[0] < x && x <= sv.dom_hi[0] && sv.dom_lo[1] < y && y <= sv.dom_hi[1] && sv.dom_lo[2] < z &&
         z <= sv.dom_hi[2]))
     return -1;
@@ -307,18 +342,22 @@ __device__ __forceinline__ int find_block(const SnapView& sv, double x, double y
     ci = ci < 0 ? 0 : (ci >= gn[d] ? gn[d] - 1 : ci);
     c[d] = ci;
   }
-  // the cell and, for points within rounding of a face, its neighbours
-  for (int dz = 0; dz <= 2; ++dz)
-    for (int dy = 0; dy <= 2; ++dy)
-      for (int dx = 0; dx <= 2; ++dx) {
-        const int cx = c[0] + (dx == 2 ? -1 : dx), cy = c[1] + (dy == 2 ? -1 : dy), cz = c[2] + (dz == 2 ? -1 : dz);
+  {
+    const int b = grid[((long)c[2] * sv.gy + c[1]) * sv.gx + c[0]];
+    if (b >= 0 && in_block(meta + kMeta * (long)b, x, y, z)) return b;
+  }
+  // within rounding of a face: the neighbouring cells
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int cx = c[0] + dx, cy = c[1] + dy, cz = c[2] + dz;
         if (cx < 0 || cy < 0 || cz < 0 || cx >= sv.gx || cy >= sv.gy || cz >= sv.gz) continue;
-        const int b = sv.grid[((long)cz * sv.gy + cy) * sv.gx + cx];
-        if (b >= 0 && in_block(sv.meta + 16 * (long)b, x, y, z)) return b;
+        const int b = grid[((long)cz * sv.gy + cy) * sv.gx + cx];
+        if (b >= 0 && in_block(meta + kMeta * (long)b, x, y, z)) return b;
       }
   int found = -1;  // holes in the tiling: scan, as the reference does
   for (int b = 0; b < sv.nmb; ++b)
-    if (in_block(sv.meta + 16 * (long)b, x, y, z)) found = b;
+    if (in_block(meta + kMeta * (long)b, x, y, z)) found = b;
   return found;
 }
 
@@ -327,83 +366,120 @@ __device__ __forceinline__ T lerp_ref(T a, T b, T t) {
   return a + (b - a) * t;
 }
 
-// Trilinear primitives at (x, y, z) in block mb (athenak.py:718-757).
-template <typename T>
-__device__ __forceinline__ void trilinear(const SnapView& sv, int mb, double x, double y, double z, T P[8]) {
-  if (mb < 0) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) P[k] = T(0);
-    return;
-  }
-  const double* m = sv.meta + 16 * (long)mb;
+// Trilinear stencil of a point: block, first cell record, fractions
+// (athenak.py:718-733: q = x - x1v[0] + dx, index floor(q/dx), fraction
+// (q/dx) mod 1, in a block padded by one ghost cell).
+struct Stencil {
+  int mb;
+  long base;  // float4 index of the (k, j, i) corner record
+  double fr[3];
+};
+
+__device__ __forceinline__ Stencil locate(const SnapView& sv, const double* meta, const int* grid, double x,
+                                          double y, double z) {
+  Stencil st;
+  st.mb = find_block(sv, meta, grid, x, y, z);
+  st.base = 0;
+  st.fr[0] = st.fr[1] = st.fr[2] = 0.0;
+  if (st.mb < 0) return st;
+  const double* m = meta + kMeta * (long)st.mb;
   const double p[3] = {x, y, z};
   int id[3];
-  T fr[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const double q = p[d] - m[6 + d] + m[9 + d];
-    const double t = q / m[9 + d];
+    const double t = (p[d] - m[6 + d]) * m[9 + d];
     const double fl = __builtin_floor(t);
     int i = (int)fl;
     i = i < 0 ? 0 : (i > sv.n2 - 2 ? sv.n2 - 2 : i);
     id[d] = i;
-    fr[d] = (T)(t - fl);
+    st.fr[d] = t - fl;
   }
   const long n2 = sv.n2;
-  const float4* c = sv.cells + 2 * (((long)mb * n2 + id[2]) * n2 * n2 + (long)id[1] * n2 + id[0]);
-  const long sj = 2 * n2, sk = 2 * n2 * n2;
+  st.base = 2 * (((long)st.mb * n2 + id[2]) * n2 * n2 + (long)id[1] * n2 + id[0]);
+  return st;
+}
+
+// The 8 corner records of a stencil: 4 rows of 64 contiguous bytes.
+struct Corners {
   float4 q[8][2];
+};
+
+__device__ __forceinline__ void fetch(const SnapView& sv, const Stencil& st, Corners& c) {
+  if (st.mb < 0) return;
+  const long n2 = sv.n2, sj = 2 * n2, sk = 2 * n2 * n2;
+  const float4* b = sv.cells + st.base;
 #pragma unroll
   for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
     for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
       for (int ii = 0; ii < 2; ++ii) {
-        const float4* e = c + kk * sk + jj * sj + 2 * ii;
-        q[kk * 4 + jj * 2 + ii][0] = e[0];
-        q[kk * 4 + jj * 2 + ii][1] = e[1];
+        const float4* e = b + kk * sk + jj * sj + 2 * ii;
+        c.q[kk * 4 + jj * 2 + ii][0] = e[0];
+        c.q[kk * 4 + jj * 2 + ii][1] = e[1];
       }
+}
+
+// Interpolate x1, then x2, then x3 (athenak.py:746-752); zero outside.
+template <typename T>
+__device__ __forceinline__ void interp(const Stencil& st, const Corners& c, T P[8]) {
+  if (st.mb < 0) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) P[k] = T(0);
+    return;
+  }
+  const T f0 = (T)st.fr[0], f1 = (T)st.fr[1], f2 = (T)st.fr[2];
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
     T s[8];
 #pragma unroll
     for (int c8 = 0; c8 < 8; ++c8) {
-      const float4 w = q[c8][v >> 2];
+      const float4 w = c.q[c8][v >> 2];
       const int r = v & 3;
       s[c8] = (T)(r == 0 ? w.x : (r == 1 ? w.y : (r == 2 ? w.z : w.w)));
     }
     // naming of the reference: d{x3}{x2}{x1}, index = x3*4 + x2*2 + x1
-    const T daa = lerp_ref(s[0], s[1], fr[0]);
-    const T dab = lerp_ref(s[2], s[3], fr[0]);
-    const T dba = lerp_ref(s[4], s[5], fr[0]);
-    const T dbb = lerp_ref(s[6], s[7], fr[0]);
-    const T da = lerp_ref(daa, dab, fr[1]);
-    const T db = lerp_ref(dba, dbb, fr[1]);
-    P[v] = lerp_ref(da, db, fr[2]);
+    const T daa = lerp_ref(s[0], s[1], f0);
+    const T dab = lerp_ref(s[2], s[3], f0);
+    const T dba = lerp_ref(s[4], s[5], f0);
+    const T dbb = lerp_ref(s[6], s[7], f0);
+    const T da = lerp_ref(daa, dab, f1);
+    const T db = lerp_ref(dba, dbb, f1);
+    P[v] = lerp_ref(da, db, f2);
   }
 }
 
-// Fluid scalars at one state (athenak.py:759-794): returns rho, u, pitch,
-// k.u, |b|.  Uses g^{t mu} = eta^{t mu} - f l^t l^mu in closed form.
+template <typename T>
+__device__ __forceinline__ void trilinear(const SnapView& sv, double x, double y, double z, T P[8]) {
+  const Stencil st = locate(sv, sv.meta, sv.grid, x, y, z);
+  Corners c;
+  fetch(sv, st, c);
+  interp<T>(st, c, P);
+}
+
+// Fluid scalars at one state (athenak.py:759-794): rho, u, cos(pitch), k.u,
+// |b|.  cos(pitch) = k.b / (|k.u| |b|), NaN -> cos(fallback), clamped to
+// [-1, 1]; the pitch angle itself is acos of it.  Uses g^{t mu} = eta^{t mu}
+// - f l^t l^mu in closed form (alpha = 1/sqrt(1 + f), g^{ti} = f l_i).
 template <typename T>
 __device__ __forceinline__ void fluid_scalars(const T P[8], T x, T y, T z, T k0, T k1, T k2, T k3, T a,
                                               T fallback_cos, T out[5]) {
   using M = Math<T>;
   const KS<T> m = ks_metric<T>(x, y, z, a);
   const T f = m.f, l1 = m.l1, l2 = m.l2, l3 = m.l3;
-  const T alpha = M::sqrt(T(1) / (T(1) + f));
+  const T alpha = M::sqrt(M::div(T(1), T(1) + f));
   const T U1 = P[2], U2 = P[3], U3 = P[4];
   const T lU = l1 * U1 + l2 * U2 + l3 * U3;
   const T gamma = M::sqrt(T(1) + (U1 * U1 + U2 * U2 + U3 * U3 + f * lU * lU));
   const T ga = gamma * alpha;
-  const T uc0 = gamma / alpha;
+  const T uc0 = M::div(gamma, alpha);
   const T uc1 = U1 - ga * (f * l1), uc2 = U2 - ga * (f * l2), uc3 = U3 - ga * (f * l3);
   const T Lu = uc0 + l1 * uc1 + l2 * uc2 + l3 * uc3;
   const T fLu = f * Lu;
   const T ul0 = -uc0 + fLu, ul1 = uc1 + l1 * fLu, ul2 = uc2 + l2 * fLu, ul3 = uc3 + l3 * fLu;
   const T B1 = P[5], B2 = P[6], B3 = P[7];
   const T bc0 = B1 * ul1 + B2 * ul2 + B3 * ul3;
-  const T iu0 = T(1) / uc0;
+  const T iu0 = M::div(T(1), uc0);
   const T bc1 = (B1 + uc1 * bc0) * iu0, bc2 = (B2 + uc2 * bc0) * iu0, bc3 = (B3 + uc3 * bc0) * iu0;
   const T Lb = bc0 + l1 * bc1 + l2 * bc2 + l3 * bc3;
   const T fLb = f * Lb;
@@ -411,40 +487,60 @@ __device__ __forceinline__ void fluid_scalars(const T P[8], T x, T y, T z, T k0,
   const T kdotu = k0 * ul0 + k1 * ul1 + k2 * ul2 + k3 * ul3;
   const T kdotb = k0 * bl0 + k1 * bl1 + k2 * bl2 + k3 * bl3;
   const T bdotb = bc0 * bl0 + bc1 * bl1 + bc2 * bl2 + bc3 * bl3;
-  T p = kdotb / (M::abs(kdotu) * M::sqrt(bdotb));
+  const T bmag = M::sqrt(bdotb);
+  T p = M::div(kdotb, M::abs(kdotu) * bmag);
   if (is_nan(p)) p = fallback_cos;
   if (M::abs(p) > T(1)) p = p / M::abs(p);
   out[0] = P[0];
   out[1] = P[1];
-  out[2] = M::acos(p);
+  out[2] = p;
   out[3] = kdotu;
-  out[4] = M::sqrt(bdotb);
+  out[4] = bmag;
 }
 
 // ------------------------------------------------------------- emission
-// transfer.py:30-86 for one sample
+// transfer.py:30-86 for one sample, given sin(pitch angle).
+//   nu_s = (2/9) (e B / 2 pi m_e c) Theta^2 sin(theta), X = nu / nu_s
+//   j = Ne nu_s (X^1/2 + 2^(11/12) X^1/6)^2 / (2 Theta^2) e^{-X^1/3} sqrt2 pi e^2 / 3c
+//   alpha = j / B_nu,  B_nu = 2 h nu^3 / (e^x - 1) / c^2,  x = h nu / m_e c^2 Theta
+// X^1/6 is taken as sqrt(X^1/3).  FP32 evaluates alpha without forming
+// nu^3, which overflows single precision for strongly blueshifted samples.
 template <typename T>
-__device__ __forceinline__ void synchrotron(T Ne, T Th, T B, T pitch, T nu, bool invariant, T rescale, T& jo,
-                                            T& ao) {
+__device__ __forceinline__ void synchrotron_sin(T Ne, T Th, T B, T sin_pitch, T nu, bool invariant, T rescale,
+                                                T& jo, T& ao) {
   using M = Math<T>;
   const T nuc = T(kEE) * B / T(2.0 * kPI * kME * kCL);
-  const T nus = T(2.0 / 9.0) * nuc * (Th * Th) * M::sin(pitch);
-  const T X = nu / nus;
-  const T var = M::exp(-M::pow(X, T(1.0 / 3)));
-  const T term = M::sqrt(X) + T(1.8877486253633870) * M::pow(X, T(1.0 / 6));  // 2^(11/12)
-  T em = Ne * nus * (term * term) / (T(2) * (Th * Th));
+  const T nus = T(2.0 / 9.0) * nuc * (Th * Th) * sin_pitch;
+  const T X = M::div(nu, nus);
+  const T c3 = M::cbrt(X);
+  const T var = M::exp(-c3);
+  const T term = M::sqrt(X) + T(1.8877486253633870) * M::sqrt(c3);  // 2^(11/12)
+  T em = M::div(Ne * nus * (term * term), T(2) * (Th * Th));
   em = em * var * T(1.4142135623730951 * kPI * kEE * kEE / (3.0 * kCL));
   if (X > T(1e12)) em = T(0);
   if (Th < T(0.3)) em = T(0);
-  const T bx = T(kHPL) * nu / (T(kME * kCL * kCL) * Th);
-  const T series = bx / T(24) * (T(24) + bx * (T(12) + bx * (T(4) + bx)));
-  const T Bden = (bx < T(2e-3)) ? series : (M::exp(bx) - T(1));
-  const T Bnu = (T(2.0 * kHPL) * (nu * nu * nu) / Bden) / T(kCL * kCL);
-  T ab = em / Bnu;
-  if (invariant) {
-    const T rn = nu * rescale;
-    em = em / (rn * rn);
-    ab = ab * rn;
+  const T bx = M::div(T(kHPL) * nu, T(kME * kCL * kCL) * Th);
+  T Bden = bx / T(24) * (T(24) + bx * (T(12) + bx * (T(4) + bx)));
+  if (!(bx < T(2e-3))) Bden = M::exp(bx) - T(1);  // rarely taken: a branch, not a select
+  T ab;
+  if constexpr (sizeof(T) == 8) {
+    const T Bnu = (T(2.0 * kHPL) * (nu * nu * nu) / Bden) / T(kCL * kCL);
+    ab = em / Bnu;
+    if (invariant) {
+      const T rn = nu * rescale;
+      em = em / (rn * rn);
+      ab = ab * rn;
+    }
+  } else {
+    // B_nu = 2 (h nu / B_den) nu^2 / c^2: h nu / B_den stays near m_e c^2 Theta
+    const T q = M::div(T(kHPL) * nu, Bden);
+    const T Bnu = T(2.0 / (kCL * kCL)) * q * (nu * nu);
+    ab = M::div(em, Bnu);
+    if (invariant) {
+      const T rn = nu * rescale;
+      em = M::div(em, rn * rn);
+      ab = ab * rn;
+    }
   }
   if (is_nan(em)) em = T(0);
   if (is_nan(ab)) ab = T(0);
@@ -452,13 +548,21 @@ __device__ __forceinline__ void synchrotron(T Ne, T Th, T B, T pitch, T nu, bool
   ao = ab;
 }
 
+// transfer.py:30-86 with the pitch angle itself
+template <typename T>
+__device__ __forceinline__ void synchrotron(T Ne, T Th, T B, T pitch, T nu, bool invariant, T rescale, T& jo,
+                                            T& ao) {
+  synchrotron_sin<T>(Ne, Th, B, Math<T>::sin(pitch), nu, invariant, rescale, jo, ao);
+}
+
 // electrons.py:32-50
 template <typename T>
 __device__ __forceinline__ T rlow_rhigh(T dens, T u, T beta, T r_low, T r_high, T ge, T gi) {
+  using M = Math<T>;
   const T b2 = beta * beta;
-  const T Tr = (r_high * b2 + r_low) / (T(1) + b2);
+  const T Tr = M::div(r_high * b2 + r_low, T(1) + b2);
   T te = T(kCL * kCL) * (T(kMP) * u * (ge - T(1)) * (gi - T(1)));
-  te = te / (dens * ((gi - T(1)) + (ge - T(1)) * Tr));
+  te = M::div(te, dens * ((gi - T(1)) + (ge - T(1)) * Tr));
   return te / T(kME * kCL * kCL);
 }
 
@@ -467,22 +571,23 @@ struct ImageParams {
   double fluid_gamma, r_high, Ne_unit, B_unit, L_unit, nu_obs, fallback_cos, sigma_cut;
 };
 
-// (j, alpha) at one accepted state (images.py:86-118)
+// (j, alpha) from interpolated primitives at one accepted state
+// (images.py:86-118).  sin(acos(p)) is taken as sqrt(1 - p^2).
 template <typename T>
-__device__ __forceinline__ void emission(const SnapView& sv, const ImageParams& ip, const double s[8], double a,
-                                         T& j, T& al) {
-  const int mb = find_block(sv, s[1], s[2], s[3]);
-  T P[8];
-  trilinear<T>(sv, mb, s[1], s[2], s[3], P);
+__device__ __forceinline__ void emission_from_prims(const ImageParams& ip, const T P[8], const double s[8], double a,
+                                                    T& j, T& al) {
+  using M = Math<T>;
   T sc[5];
   fluid_scalars<T>(P, (T)s[1], (T)s[2], (T)s[3], (T)s[4], (T)s[5], (T)s[6], (T)s[7], (T)a, (T)ip.fallback_cos, sc);
-  const T dens = sc[0], u = sc[1], pitch = sc[2], kdotu = sc[3], b = sc[4];
+  const T dens = sc[0], u = sc[1], cp = sc[2], kdotu = sc[3], b = sc[4];
   const T bsq = b * b;
-  const T beta = u * (T(ip.fluid_gamma) - T(1)) / bsq / T(0.5);
-  const T sigma = bsq / dens;
+  const T beta = M::div(u * (T(ip.fluid_gamma) - T(1)), bsq) / T(0.5);
+  const T sigma = M::div(bsq, dens);
   const T Th = rlow_rhigh<T>(dens, u, beta, T(1), T(ip.r_high), T(4.0 / 3), T(5.0 / 3));
   const T Ne = T(ip.Ne_unit) * dens, Bg = T(ip.B_unit) * b, nu = -kdotu * T(ip.nu_obs);
-  synchrotron<T>(Ne, Th, Bg, pitch, nu, true, T(1.0 / ip.nu_obs), j, al);
+  const T one_m = T(1) - cp * cp;
+  const T sp = M::sqrt(one_m > T(0) ? one_m : T(0));
+  synchrotron_sin<T>(Ne, Th, Bg, sp, nu, true, T(1.0 / ip.nu_obs), j, al);
   if (sigma > T(ip.sigma_cut)) {
     j = T(0);
     al = T(0);

@@ -14,4 +14,8 @@ struct RenderDev {
 
 int fill_render_dev(const mk_render_params* p, RenderDev* rp);
 
+// LDS budget for the staged block geometry + lookup grid (bytes)
+constexpr size_t kMaxTableLds = 64 * 1024;
+size_t table_bytes(const SnapView& sv);
+
 }  // namespace mk

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "mk_device.h"
 #include "mk_internal.h"
@@ -23,9 +24,15 @@
 
 namespace mk {
 
+// One ray: the sample of accepted step k is located (lookup tables in LDS)
+// and its 8 corner records are requested right after the step, then
+// consumed after the next RK4 step has been computed, so the gather latency
+// overlaps the geodesic arithmetic.  Contributions are still added in step
+// order, so the result is the same as sampling in place.
 template <bool MIXED>
-__device__ __forceinline__ void trace_ray(const SnapView& sv, const RenderDev& rp, double x_img, double y_img,
-                                          double& I_out, int& steps_out) {
+__device__ __forceinline__ void trace_ray(const SnapView& sv, const double* meta, const int* grid, const RenderDev& rp,
+                                          double x_img, double y_img, double& I_out, int& steps_out) {
+  using T = typename std::conditional<MIXED, float, double>::type;
   double sx[4], sv4[4], s[8];
   camera_ray(x_img, y_img, rp.ci, rp.si, rp.dist, sx, sv4);
   nullify(sx, sv4, rp.a, s);
@@ -33,6 +40,18 @@ __device__ __forceinline__ void trace_ray(const SnapView& sv, const RenderDev& r
   double dt = step_rule(r, rp.rEH, rp.div, rp.tol);
   double I = 0.0, trans = 1.0, drdl = 1e300;
   int nacc = 0;
+  bool pend = false;
+  Stencil pst;
+  Corners pc;
+  double ps[8], pdt = 0.0;
+  auto consume = [&]() {
+    T P[8], j, al;
+    interp<T>(pst, pc, P);
+    emission_from_prims<T>(rp.ip, P, ps, rp.a, j, al);
+    const double c = -pdt * rp.ip.L_unit;
+    I += trans * (c * (double)j);
+    trans *= (1.0 - c * (double)al);
+  };
   for (int k = 0; k < rp.N; ++k) {
     if (dt == 0.0) break;
     double nw[8];
@@ -43,23 +62,21 @@ __device__ __forceinline__ void trace_ray(const SnapView& sv, const RenderDev& r
       if (fp64) rk4_step<double>(s, dt, rp.a, nw);
       else rk4_step_mixed(s, dt, (float)rp.a, nw);
     }
+    if (pend) {
+      consume();
+      pend = false;
+    }
     const double rn = radius_cal(nw[1], nw[2], nw[3], rp.a);
     const double dtn = step_rule(rn, rp.rEH, rp.div, rp.tol);
     if (dtn == 0.0) break;
     ++nacc;
-    if (k + 1 < rp.N) {
-      const double c = -dt * rp.ip.L_unit;
-      if constexpr (!MIXED) {
-        double j, al;
-        emission<double>(sv, rp.ip, nw, rp.a, j, al);
-        I += trans * (c * j);
-        trans *= (1.0 - c * al);
-      } else {
-        float j, al;
-        emission<float>(sv, rp.ip, nw, rp.a, j, al);
-        I += trans * (c * (double)j);
-        trans *= (1.0 - c * (double)al);
-      }
+    if (k + 1 < rp.N) {  // row N is never emitted
+      pst = locate(sv, meta, grid, nw[1], nw[2], nw[3]);
+      fetch(sv, pst, pc);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) ps[q] = nw[q];
+      pdt = dt;
+      pend = true;
     }
     drdl = (rn - r) / dt;
 #pragma unroll
@@ -67,6 +84,7 @@ __device__ __forceinline__ void trace_ray(const SnapView& sv, const RenderDev& r
     dt = dtn;
     r = rn;
   }
+  if (pend) consume();
   I_out = I;
   steps_out = nacc;
 }
@@ -87,17 +105,42 @@ __device__ __forceinline__ bool pixel_of_lane(const RenderDev& rp, int& i, int& 
   return i < rp.res && j < rp.res;
 }
 
-template <bool MIXED>
+// Copies the block geometry and the leaf lookup grid into LDS (when they
+// fit), so block lookup and stencil arithmetic never wait on HBM.
+template <bool LDS>
+__device__ __forceinline__ void stage_tables(const SnapView& sv, double* lds, const double*& meta, const int*& grid) {
+  meta = sv.meta;
+  grid = sv.grid;
+  if constexpr (LDS) {
+    const int nm = sv.nmb * kMeta, ng = sv.gx * sv.gy * sv.gz;
+    for (int t = threadIdx.x; t < nm; t += blockDim.x) lds[t] = sv.meta[t];
+    int* g = reinterpret_cast<int*>(lds + nm);
+    for (int t = threadIdx.x; t < ng; t += blockDim.x) g[t] = sv.grid[t];
+    __syncthreads();
+    meta = lds;
+    grid = g;
+  }
+}
+
+template <bool MIXED, bool LDS>
 __global__ void __launch_bounds__(256) k_render(SnapView sv, RenderDev rp, double* __restrict__ img,
                                                 int32_t* __restrict__ steps) {
+  extern __shared__ double lds[];
+  const double* meta;
+  const int* grid;
+  stage_tables<LDS>(sv, lds, meta, grid);
   int i, j;
   if (!pixel_of_lane(rp, i, j)) return;
   double I;
   int n;
-  trace_ray<MIXED>(sv, rp, grid_coord(rp.lo, rp.step, i), grid_coord(rp.lo, rp.step, j), I, n);
+  trace_ray<MIXED>(sv, meta, grid, rp, grid_coord(rp.lo, rp.step, i), grid_coord(rp.lo, rp.step, j), I, n);
   const long p = (long)i * rp.res + j;
   img[p] = I;
   if (steps) steps[p] = n;
+}
+
+size_t table_bytes(const SnapView& sv) {
+  return (size_t)sv.nmb * kMeta * sizeof(double) + (size_t)sv.gx * sv.gy * sv.gz * sizeof(int);
 }
 
 int fill_render_dev(const mk_render_params* p, RenderDev* rp) {
@@ -150,10 +193,18 @@ extern "C" int mk_render(const mk_snapshot* snap, const mk_render_params* p, dou
   const int waves_per_wg = 4;
   long nwg = (rp.ntiles + waves_per_wg - 1) / waves_per_wg;
   nwg = (nwg + 7) / 8 * 8;
-  if (p->precision == 0)
-    k_render<false><<<(unsigned)nwg, 64 * waves_per_wg, 0, as_stream(stream)>>>(snap->view(), rp, d_image, d_steps);
-  else
-    k_render<true><<<(unsigned)nwg, 64 * waves_per_wg, 0, as_stream(stream)>>>(snap->view(), rp, d_image, d_steps);
+  const SnapView sv = snap->view();
+  const size_t lds = table_bytes(sv);
+  const bool use_lds = lds <= kMaxTableLds;
+  const dim3 grid((unsigned)nwg), block(64 * waves_per_wg);
+  hipStream_t st = as_stream(stream);
+  if (p->precision == 0) {
+    if (use_lds) k_render<false, true><<<grid, block, lds, st>>>(sv, rp, d_image, d_steps);
+    else k_render<false, false><<<grid, block, 0, st>>>(sv, rp, d_image, d_steps);
+  } else {
+    if (use_lds) k_render<true, true><<<grid, block, lds, st>>>(sv, rp, d_image, d_steps);
+    else k_render<true, false><<<grid, block, 0, st>>>(sv, rp, d_image, d_steps);
+  }
   MK_LAUNCH_CHECK();
   return MK_OK;
 }

@@ -140,7 +140,7 @@ __global__ void k_block_index(SnapView sv, const double* S, long n, int stride, 
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const double* q = S + (long)stride * p;
-  mb[p] = find_block(sv, q[1], q[2], q[3]);
+  mb[p] = find_block(sv, sv.meta, sv.grid, q[1], q[2], q[3]);
 }
 
 __global__ void __launch_bounds__(256) k_sample_prims(SnapView sv, const double* S, long n, double* out) {
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(256) k_sample_prims(SnapView sv, const double*
   if (p >= n) return;
   const double* q = S + 8 * p;
   double P[8];
-  trilinear<double>(sv, find_block(sv, q[1], q[2], q[3]), q[1], q[2], q[3], P);
+  trilinear<double>(sv, q[1], q[2], q[3], P);
 #pragma unroll
   for (int c = 0; c < 8; ++c) out[c * n + p] = P[c];
 }
@@ -161,8 +161,9 @@ __global__ void __launch_bounds__(256) k_sample_scalars(SnapView sv, const doubl
 #pragma unroll
   for (int k = 0; k < 8; ++k) q[k] = S[8 * p + k];
   double P[8], sc[5];
-  trilinear<double>(sv, find_block(sv, q[1], q[2], q[3]), q[1], q[2], q[3], P);
+  trilinear<double>(sv, q[1], q[2], q[3], P);
   fluid_scalars<double>(P, q[1], q[2], q[3], q[4], q[5], q[6], q[7], a, fbc, sc);
+  sc[2] = ::acos(sc[2]);  // pitch angle from its cosine
 #pragma unroll
   for (int c = 0; c < 5; ++c) out[c * n + p] = sc[c];
 }
@@ -261,16 +262,21 @@ int mk_snapshot_create(const float* h_prims, const int32_t* h_var, long nmb, int
           }
         }
   }
-  // per-block geometry: lo, hi (faces), v0 (first centre), dx = v1 - v0
-  std::vector<double> meta((size_t)nmb * 16, 0.0);
+  // per-block geometry: lo, hi (faces), q0 = v0 - dx, 1 / dx (dx = v1 - v0)
+  std::vector<double> meta((size_t)nmb * kMeta, 0.0);
   const double* f[3] = {x1f, x2f, x3f};
   const double* v[3] = {x1v, x2v, x3v};
   for (long b = 0; b < nmb; ++b)
     for (int ax = 0; ax < 3; ++ax) {
-      meta[16 * b + ax] = f[ax][b * (n + 1)];
-      meta[16 * b + 3 + ax] = f[ax][b * (n + 1) + n];
-      meta[16 * b + 6 + ax] = v[ax][b * n];
-      meta[16 * b + 9 + ax] = v[ax][b * n + 1] - v[ax][b * n];
+      const double dx = v[ax][b * n + 1] - v[ax][b * n];
+      if (!(dx > 0)) {
+        set_error("mk_snapshot_create: block %ld has non-increasing cell centres", b);
+        return MK_ERR_MESH;
+      }
+      meta[kMeta * b + ax] = f[ax][b * (n + 1)];
+      meta[kMeta * b + 3 + ax] = f[ax][b * (n + 1) + n];
+      meta[kMeta * b + 6 + ax] = v[ax][b * n] - dx;
+      meta[kMeta * b + 9 + ax] = 1.0 / dx;
     }
   // lookup grid at the finest level's block size
   mk_snapshot* s = new mk_snapshot();
@@ -283,10 +289,10 @@ int mk_snapshot_create(const float* h_prims, const int32_t* h_var, long nmb, int
   for (int ax = 0; ax < 3; ++ax) {
     double lo = meta[ax], hi = meta[3 + ax];
     for (long b = 0; b < nmb; ++b) {
-      lo = std::min(lo, meta[16 * b + ax]);
-      hi = std::max(hi, meta[16 * b + 3 + ax]);
+      lo = std::min(lo, meta[kMeta * b + ax]);
+      hi = std::max(hi, meta[kMeta * b + 3 + ax]);
     }
-    const double w = meta[16 * bmax + 3 + ax] - meta[16 * bmax + ax];
+    const double w = meta[kMeta * bmax + 3 + ax] - meta[kMeta * bmax + ax];
     if (!(w > 0)) {
       delete s;
       set_error("mk_snapshot_create: degenerate block extent");
@@ -308,8 +314,8 @@ int mk_snapshot_create(const float* h_prims, const int32_t* h_var, long nmb, int
   for (long b = 0; b < nmb; ++b) {
     long c0[3], c1[3];
     for (int ax = 0; ax < 3; ++ax) {
-      c0[ax] = std::llround((meta[16 * b + ax] - s->g0[ax]) * s->ginv[ax]);
-      c1[ax] = std::llround((meta[16 * b + 3 + ax] - s->g0[ax]) * s->ginv[ax]);
+      c0[ax] = std::llround((meta[kMeta * b + ax] - s->g0[ax]) * s->ginv[ax]);
+      c1[ax] = std::llround((meta[kMeta * b + 3 + ax] - s->g0[ax]) * s->ginv[ax]);
       c0[ax] = std::max(0L, c0[ax]);
       c1[ax] = std::min((long)s->G[ax], c1[ax]);
     }

@@ -16,13 +16,16 @@ from mahakala_amd import _lib
 from mahakala_amd.constants import Msun
 from mahakala_amd.electrons import rlow_rhigh_model
 
-PRECISIONS = {"plain64": 0, "fp64": 0, "mixed": 1}
+PRECISIONS = {"plain64": 0, "fp64": 0, "mixed": 1, "mixed-zone": 1}
 
-# FP64 zone of the mixed kernel: substeps inside r < 4 M, and near the
-# photon ring (r < 5 M with |dr/dlambda| < 0.1)
-MIXED_FP64_RADIUS = 4.0
-MIXED_RING_RADIUS = 5.0
-MIXED_RING_DRDL = 0.1
+# Mixed precision keeps in FP64 the ray state and its RK4 accumulation, the
+# step-size / termination rule, the camera-ray null normalisation and the
+# optical-depth / intensity accumulation; the RK4 right-hand sides, sampling
+# and emission run in FP32.  "mixed-zone" also takes the RK4 substeps in FP64
+# inside r < 4 M and near the photon ring (r < 5 M with |dr/dlambda| < 0.1).
+# On the 256^2 torus image the zone changes the total flux by < 1e-7
+# relative and costs ~30% in time, so plain "mixed" leaves it off.
+ZONES = {"mixed": (0.0, 0.0, 0.0), "mixed-zone": (4.0, 5.0, 0.1)}
 
 
 def render_params(bhspin, camera_inclination=60, camera_distance=1000, mass_scale=1.e26, M_bh=6.2e9 * Msun,
@@ -45,9 +48,10 @@ def render_params(bhspin, camera_inclination=60, camera_distance=1000, mass_scal
     p.tol = 1e-4
     p.fallback_pitch = np.pi / 3.
     p.sigma_cut = 100.0
-    p.fp64_radius = MIXED_FP64_RADIUS if fp64_radius is None else float(fp64_radius)
-    p.ring_radius = MIXED_RING_RADIUS if ring_radius is None else float(ring_radius)
-    p.ring_drdl = MIXED_RING_DRDL if ring_drdl is None else float(ring_drdl)
+    zr, zring, zdrdl = ZONES.get(precision, (0.0, 0.0, 0.0))
+    p.fp64_radius = zr if fp64_radius is None else float(fp64_radius)
+    p.ring_radius = zring if ring_radius is None else float(ring_radius)
+    p.ring_drdl = zdrdl if ring_drdl is None else float(ring_drdl)
     p.resolution = int(resolution)
     p.max_nsteps = int(max_nsteps)
     p.precision = PRECISIONS[precision]
