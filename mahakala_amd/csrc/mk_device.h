@@ -107,10 +107,19 @@ __device__ __forceinline__ KS<T> ks_metric(T x, T y, T z, T a) {
   return m;
 }
 
-// dv/dlambda = -Gamma^alpha_{mu nu} v^mu v^nu for g = eta + f l l.
+// dv/dlambda = -Gamma^alpha_{mu nu} v^mu v^nu for g = eta + f l l
+// (geodesics.py:294-309: g^{ab} (-d_s g_{bn} v^s v^n + 1/2 d_b g_{mn} v^m v^n)).
 // With L = l.v, W_i = v^s d_s l_i, D_s = d_s l_mu v^mu, F_s = d_s f:
 //   c_b = 1/2 F_b L^2 + f L D_b - l_b (L F.v + f W.v) - f L W_b
 //   a^alpha = eta^{alpha b} c_b - f l^alpha (l^b c_b),  l^mu = (-1, l_i).
+// The Jacobian of l_i is d_j l_i = A_i d_j(r^2) + B_ij with
+//   A = (s (x/2r - l1), s (y/2r - l2), -l3 / 2r^2),  s = 1/(r^2 + a^2),
+//   B = [[s r, s a, 0], [-s a, s r, 0], [0, 0, 1/r]],
+// so W, D and W.v reduce to two dot products (dv = d(r^2).v, Av = A.v):
+//   D - W = d(r^2) Av - A dv + (-2 s a v2, 2 s a v1, 0),
+//   W.v = dv Av + s r (v1^2 + v2^2) + v3^2 / r,
+// and F = kappa d(r^2) - zeta e_z, kappa = f (3/2r^2 - 2 r^2/den),
+// zeta = 2 f a^2 z / den, den = r^4 + a^2 z^2.
 template <typename T>
 __device__ __forceinline__ void ks_accel(T x, T y, T z, T v0, T v1, T v2, T v3, T a, T acc[4]) {
   using M = Math<T>;
@@ -121,48 +130,35 @@ __device__ __forceinline__ void ks_accel(T x, T y, T z, T v0, T v1, T v2, T v3, 
   const T rr = Q + kk;
   const T r = M::sqrt(rr);
   const T ir = M::rcp(r);
-  const T den = rr * rr + aa * zz;
-  const T iden = M::rcp(den);
+  const T iden = M::rcp(rr * rr + aa * zz);
   const T f = (T(2) * rr * r) * iden;
   const T s = M::rcp(rr + aa);
   const T l1 = (r * x + a * y) * s;
   const T l2 = (r * y - a * x) * s;
   const T l3 = z * ir;
-  // d rr/dx_i and d r/dx_i
+  // gradient of r^2, and A
   const T g = rr * iQ;
-  const T drr1 = x * g, drr2 = y * g, drr3 = z * (rr + aa) * iQ;
+  const T d1 = x * g, d2 = y * g, d3 = z * (rr + aa) * iQ;
   const T h = T(0.5) * ir;
-  const T dr1 = drr1 * h, dr2 = drr2 * h, dr3 = drr3 * h;
-  // d f = f (3 dr / r - d den / den),  d den = 2 rr d rr + 2 a^2 z e_z
-  const T c3 = T(3) * ir, e2 = T(2) * rr * iden;
-  const T df1 = f * (c3 * dr1 - e2 * drr1);
-  const T df2 = f * (c3 * dr2 - e2 * drr2);
-  const T df3 = f * (c3 * dr3 - e2 * drr3 - T(2) * aa * z * iden);
-  // d l_i / d x_j
-  const T d11 = s * (dr1 * x + r - l1 * drr1), d12 = s * (dr2 * x + a - l1 * drr2), d13 = s * (dr3 * x - l1 * drr3);
-  const T d21 = s * (dr1 * y - a - l2 * drr1), d22 = s * (dr2 * y + r - l2 * drr2), d23 = s * (dr3 * y - l2 * drr3);
-  const T d31 = -l3 * dr1 * ir, d32 = -l3 * dr2 * ir, d33 = (T(1) - l3 * dr3) * ir;
+  const T A1 = s * (h * x - l1), A2 = s * (h * y - l2), A3 = -l3 * h * ir;
+  const T kap = f * (T(3) * h * ir - T(2) * rr * iden);
+  const T zeta = T(2) * f * aa * z * iden;
   const T L = v0 + l1 * v1 + l2 * v2 + l3 * v3;
-  const T Fv = df1 * v1 + df2 * v2 + df3 * v3;
-  const T W1 = d11 * v1 + d12 * v2 + d13 * v3;
-  const T W2 = d21 * v1 + d22 * v2 + d23 * v3;
-  const T W3 = d31 * v1 + d32 * v2 + d33 * v3;
-  const T D1 = d11 * v1 + d21 * v2 + d31 * v3;
-  const T D2 = d12 * v1 + d22 * v2 + d32 * v3;
-  const T D3 = d13 * v1 + d23 * v2 + d33 * v3;
-  const T Wv = W1 * v1 + W2 * v2 + W3 * v3;
-  const T P = L * Fv + f * Wv;
+  const T dv = d1 * v1 + d2 * v2 + d3 * v3;
+  const T Av = A1 * v1 + A2 * v2 + A3 * v3;
+  const T Wv = dv * Av + s * r * (v1 * v1 + v2 * v2) + ir * (v3 * v3);
+  const T P = L * (kap * dv - zeta * v3) + f * Wv;
   const T fL = f * L, hL2 = T(0.5) * L * L;
-  const T c0 = -P;
-  const T c1 = hL2 * df1 + fL * (D1 - W1) - l1 * P;
-  const T c2 = hL2 * df2 + fL * (D2 - W2) - l2 * P;
-  const T c3b = hL2 * df3 + fL * (D3 - W3) - l3 * P;
-  const T lc = -c0 + l1 * c1 + l2 * c2 + l3 * c3b;
-  const T flc = f * lc;
-  acc[0] = -c0 + flc;
+  const T mu = hL2 * kap + fL * Av, nu = fL * dv;
+  const T sa2 = T(2) * s * a * fL;
+  const T c1 = mu * d1 - nu * A1 - sa2 * v2 - l1 * P;
+  const T c2 = mu * d2 - nu * A2 + sa2 * v1 - l2 * P;
+  const T c3 = mu * d3 - nu * A3 - l3 * P - hL2 * zeta;
+  const T flc = f * (P + l1 * c1 + l2 * c2 + l3 * c3);
+  acc[0] = P + flc;
   acc[1] = c1 - l1 * flc;
   acc[2] = c2 - l2 * flc;
-  acc[3] = c3b - l3 * flc;
+  acc[3] = c3 - l3 * flc;
 }
 
 // RK4 over the 8-vector state, all in T (geodesics.py:317-336).
@@ -232,6 +228,23 @@ __device__ __forceinline__ double radius_cal(double x, double y, double z, doubl
   const double R = __builtin_sqrt(x * x + y * y + z * z);
   const double R2 = R * R, aa = a * a;
   return __builtin_sqrt((R2 - aa + __builtin_sqrt((R2 - aa) * (R2 - aa) + 4.0 * aa * (z * z))) / 2.0);
+}
+
+// radius_cal with R^2 = x^2 + y^2 + z^2 taken directly and the refined
+// hardware square root (within an ulp or two of radius_cal)
+__device__ __forceinline__ double radius_fast(double x, double y, double z, double a) {
+  using M = Math<double>;
+  const double R2 = x * x + y * y + z * z, aa = a * a;
+  const double w = R2 - aa;
+  return M::sqrt(0.5 * (w + M::sqrt(w * w + 4.0 * aa * (z * z))));
+}
+
+// step rule with the division by div replaced by a product with 1/div
+__device__ __forceinline__ double step_rule_fast(double r, double rEH, double inv_div, double div, double tol) {
+  const double dt = -(r - rEH) * inv_div;
+  const double m = __builtin_fabs(dt) * div;
+  if (dt != dt || m < tol || m > 1500.0) return 0.0;
+  return dt;
 }
 
 // geodesics.py:249-252: dt = -(r - r_EH)/div, zeroed on NaN, |dt| div < tol

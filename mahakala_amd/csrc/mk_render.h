@@ -5,7 +5,7 @@
 namespace mk {
 
 struct RenderDev {
-  double a, ci, si, dist, lo, step, rEH, div, tol;
+  double a, ci, si, dist, lo, step, rEH, div, inv_div, tol;
   int res, N, tiles_x;
   long ntiles;
   ImageParams ip;

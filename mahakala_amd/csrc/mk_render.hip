@@ -44,21 +44,27 @@ __device__ __forceinline__ void trace_ray(const SnapView& sv, const double* meta
   Stencil pst;
   Corners pc;
   double ps[8], pdt = 0.0;
+  // Outside the domain (all primitives zero) and wherever the density is
+  // zero, the reference's j and alpha come out 0 (or NaN, then 0), and the
+  // step adds exactly nothing: those samples skip the emission arithmetic.
   auto consume = [&]() {
+    if (pst.mb < 0) return;
     T P[8], j, al;
     interp<T>(pst, pc, P);
+    if (P[0] == T(0)) return;
     emission_from_prims<T>(rp.ip, P, ps, rp.a, j, al);
     const double c = -pdt * rp.ip.L_unit;
     I += trans * (c * (double)j);
     trans *= (1.0 - c * (double)al);
   };
+  const bool zone = MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0);
   for (int k = 0; k < rp.N; ++k) {
     if (dt == 0.0) break;
     double nw[8];
     if constexpr (!MIXED) {
       rk4_step<double>(s, dt, rp.a, nw);
     } else {
-      const bool fp64 = (r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl);
+      const bool fp64 = zone && ((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
       if (fp64) rk4_step<double>(s, dt, rp.a, nw);
       else rk4_step_mixed(s, dt, (float)rp.a, nw);
     }
@@ -66,8 +72,14 @@ __device__ __forceinline__ void trace_ray(const SnapView& sv, const double* meta
       consume();
       pend = false;
     }
-    const double rn = radius_cal(nw[1], nw[2], nw[3], rp.a);
-    const double dtn = step_rule(rn, rp.rEH, rp.div, rp.tol);
+    double rn, dtn;
+    if constexpr (!MIXED) {
+      rn = radius_cal(nw[1], nw[2], nw[3], rp.a);
+      dtn = step_rule(rn, rp.rEH, rp.div, rp.tol);
+    } else {
+      rn = radius_fast(nw[1], nw[2], nw[3], rp.a);
+      dtn = step_rule_fast(rn, rp.rEH, rp.inv_div, rp.div, rp.tol);
+    }
     if (dtn == 0.0) break;
     ++nacc;
     if (k + 1 < rp.N) {  // row N is never emitted
@@ -78,7 +90,7 @@ __device__ __forceinline__ void trace_ray(const SnapView& sv, const double* meta
       pdt = dt;
       pend = true;
     }
-    drdl = (rn - r) / dt;
+    if (zone) drdl = (rn - r) / dt;
 #pragma unroll
     for (int q = 0; q < 8; ++q) s[q] = nw[q];
     dt = dtn;
@@ -157,6 +169,7 @@ int fill_render_dev(const mk_render_params* p, RenderDev* rp) {
   rp->step = (p->fov / 2.0 - (-p->fov / 2.0)) / (double)(2 * p->resolution);
   rp->rEH = 1.0 + std::sqrt(1.0 - p->bhspin * p->bhspin);
   rp->div = p->div;
+  rp->inv_div = 1.0 / p->div;
   rp->tol = p->tol;
   rp->res = p->resolution;
   rp->N = p->max_nsteps;
