@@ -700,6 +700,31 @@ def sample_prims(snap, S):
     return prims, mb
 
 
+def four_vectors(snap, S):
+    """u^mu, u_mu, b^mu, b_mu and the metric at the points of S (athenak.py:759-781)."""
+    S = np.asarray(S, dtype=np.float64)
+    iU1, iB1 = snap.idx("velx"), snap.idx("bcc1")
+    prims, _ = sample_prims(snap, S)
+    with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+        gcov = metric(S[..., :4], snap.bhspin)
+        gcon = np.linalg.inv(gcov)
+        alpha = np.sqrt(1. / (-gcon[..., 0, 0]))
+        U = prims[..., iU1:iU1 + 3]
+        gamma = np.sqrt(1 + np.einsum("...j,...j->...", np.einsum("...i,...ij->...j", U, gcov[..., 1:, 1:]), U))
+        ucon = np.empty(S.shape[:-1] + (4,))
+        ucon[..., 0] = gamma / alpha
+        for c in range(3):
+            ucon[..., c + 1] = U[..., c] - gamma * alpha * gcon[..., 0, c + 1]
+        ucov = np.einsum("...ij,...j->...i", gcov, ucon)
+        Bp = prims[..., iB1:iB1 + 3]
+        bcon = np.empty_like(ucon)
+        bcon[..., 0] = np.einsum("...i,...i->...", Bp, ucov[..., 1:])
+        for c in range(3):
+            bcon[..., c + 1] = (Bp[..., c] + ucon[..., c + 1] * bcon[..., 0]) / ucon[..., 0]
+        bcov = np.einsum("...ij,...j->...i", gcov, bcon)
+    return ucon, ucov, bcon, bcov, gcov
+
+
 def get_fluid_scalars_from_geodesics(snap, S, fallback_pitch_angle=np.pi / 3.):
     S = np.asarray(S, dtype=np.float64)
     iU1, iU2, iU3 = snap.idx("velx"), snap.idx("vely"), snap.idx("velz")
