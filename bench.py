@@ -71,6 +71,65 @@ def time_launches(L, fn, steps, warmup):
     return np.array(ms), wall
 
 
+def serve_stream(models, precision, seconds, rate, seed=0, log=None):
+    """Poisson stream of image requests against resident snapshots through
+    the request server: resolution uniform in {64, 128, 256, 512}^2, spin
+    U[-0.98, 0.98], inclination U[5, 85] deg, frequency in {86, 230, 345} GHz,
+    snapshot uniform over the resident ones.  Returns throughput and
+    submit-to-result latency percentiles."""
+    import threading
+    from mahakala_amd.serving import Server
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n_req = max(1, int(round(seconds * rate)))
+    t_arr = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    res = rng.choice([64, 128, 256, 512], n_req)
+    spin = rng.uniform(-0.98, 0.98, n_req)
+    inc = rng.uniform(5.0, 85.0, n_req)
+    nu = rng.choice([86e9, 230e9, 345e9], n_req)
+    snap = rng.integers(0, len(models), n_req)
+    with Server(precision, use_graph=True) as srv:
+        sids = [srv.add_snapshot(m) for m in models]
+        # one small request first so that the first timed request does not
+        # pay for lazy initialisation
+        srv.result(srv.submit_image(sids[0], 0.5, 30.0, resolution=16), timeout=120)
+        srv.poll()
+        st0 = srv.stats()
+        ids = []
+        t0 = time.perf_counter()
+
+        def submitter():
+            for k in range(n_req):
+                dt = t0 + t_arr[k] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids.append(srv.submit_image(sids[snap[k]], float(spin[k]), float(inc[k]), resolution=int(res[k]),
+                                            observing_frequency=float(nu[k]), fluid_gamma=13. / 9))
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        srv.wait(-1, timeout=600)
+        wall = time.perf_counter() - t0
+        done = dict(srv.poll(cap=n_req + 16))
+        st = srv.stats()
+    lat = np.array([done[i] for i in ids])
+    rays = float(np.sum(res.astype(np.float64) ** 2))
+    out = {"workload": f"Poisson arrivals at {rate:g} req/s for {seconds:g} s ({n_req} image requests), "
+                       "resolution uniform in {64,128,256,512}^2, spin U[-0.98,0.98], inclination U[5,85] deg, "
+                       f"frequency in {{86,230,345}} GHz, {len(models)} resident 256^3-cell torus snapshots",
+           "precision": precision, "requests": n_req, "rays": int(rays), "wall_s": round(wall, 3),
+           "rays_per_s": round(rays / wall, 1), "offered_rays_per_s": round(rays / t_arr[-1], 1),
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                          "max": round(float(lat.max()), 3)},
+           "gpu_busy_frac": round((st["gpu_ms"] - st0["gpu_ms"]) / 1e3 / wall, 4),
+           "launches": int(st["iterations"] - st0["iterations"])}
+    if log:
+        print(f"[bench] serving {precision}: {out['rays_per_s'] / 1e6:.2f} M rays/s, p50 {out['latency_ms']['p50']} ms, "
+              f"p99 {out['latency_ms']['p99']} ms, busy {out['gpu_busy_frac']}", file=log, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +139,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--only", choices=["mixed", "plain64", "mixed-zone"], default=None,
                     help="time one precision only (profiling runs)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0, help="length of the Poisson serving stream")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="mean requests/s of the serving stream")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the device copy kernel over MIB MiB 3 times (traffic counter calibration)")
     args = ap.parse_args()
@@ -177,10 +239,10 @@ def main():
         "ms_per_step": round(1e3 * wall / args.steps, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32+fp64" if head == "mixed" else "fp64", "data": "synthetic",
         "config": {"workload": "256x256 image, a=0.5, i=30deg, 230 GHz, analytic torus on 512 meshblocks of 32^3 "
-                               "FP32 cells over +-40 M (configs[1])",
+                               "FP32 cells over +-40 M",
                    "resolution": res, "rays_per_step": n, "precision": head, "max_nsteps": 10000},
-        "latency_ms": {"p50": round(float(np.percentile(ms_head, 50)), 4),
-                       "p99": round(float(np.percentile(ms_head, 99)), 4), "what": "one 256^2 image per launch"},
+        "image_launch_ms": {"p50": round(float(np.percentile(ms_head, 50)), 4),
+                            "p99": round(float(np.percentile(ms_head, 99)), 4), "what": "one 256^2 image per launch"},
         "roofline": roofline,
         "accuracy_mixed_vs_plain": {k: (round(v, 9) if isinstance(v, float) else v) for k, v in acc.items()},
         "accepted_steps": int(total_steps), "mean_steps_per_ray": round(total_steps / n, 2),
@@ -202,6 +264,15 @@ def main():
                                  "speedup_over_plain": round(pw / zw, 3),
                                  "accuracy_vs_plain": {k: (round(v, 9) if isinstance(v, float) else v)
                                                        for k, v in accuracy(I_mix["mixed-zone"]).items()}}
+
+    if not args.no_serve:
+        ds1 = config2_snapshot(a, seed=1)
+        model1 = AthenakFluidModel.from_datasets(ds1, a, fluid_gamma=13. / 9)
+        sv = serve_stream([model, model1], head if head != "mixed-zone" else "mixed", args.serve_seconds,
+                          args.serve_rate, log=sys.stderr)
+        out["serving"] = sv
+        out["latency_ms"] = dict(sv["latency_ms"], what="submit-to-result latency of the serving stream")
+        del model1, ds1
 
     if not args.no_cpu:
         from oracle import c_oracle

@@ -182,6 +182,66 @@ typedef struct mk_render_params {
 int mk_render(const mk_snapshot* snap, const mk_render_params* params, double* d_image, int32_t* d_steps,
               void* stream);
 
+/* ---------------------------------------------------------------- serving */
+/* Continuous batching of image and shadow requests on one GPU.  No single
+ * reference counterpart: the reference renders one image per make_image
+ * call (images.py:30-144) and one bisection per find_shadow_bisection_angles
+ * call (geodesics.py:405-435); the server runs both kinds of work, for many
+ * requests of different size, spin, inclination and frequency, in shared
+ * launches of packed rays.  A worker thread admits queued requests while
+ * their output buffers fit the HBM budget, and launches an advance kernel in
+ * which every lane runs up to steps_per_launch RK4 steps and, when its ray or
+ * bisection finishes, takes the next work item of any admitted request.
+ * Results do not depend on what else is in flight. */
+typedef struct mk_server mk_server;
+
+typedef struct mk_server_config {
+  int32_t precision;        /* image rays: 0 plain FP64, 1 mixed (bisection is FP64) */
+  int32_t lanes;            /* ray slots = lanes of the advance kernel; 0 = 2 waves per SIMD */
+  int32_t steps_per_launch; /* RK4 steps per lane per launch; 0 = 256 */
+  int32_t max_requests;     /* requests in flight; 0 = 1024 */
+  size_t hbm_budget;        /* device bytes for in-flight outputs; 0 = 4 GiB */
+  int32_t use_graph;        /* replay the per-iteration launch as a hipGraph */
+  int32_t flags;            /* reserved, 0 */
+} mk_server_config;
+
+typedef struct mk_image_request {
+  int32_t snapshot;         /* id from mk_server_add_snapshot */
+  int32_t resolution;
+  int32_t max_nsteps;       /* 10000 in make_image */
+  int32_t reserved;
+  double bhspin, inclination, distance, fov, observing_frequency, mass_scale, M_bh, r_high, fluid_gamma;
+} mk_image_request;
+
+typedef struct mk_shadow_request {
+  double bhspin, inclination, distance, error_allowed;
+  int32_t n_angles, max_steps, max_it, reserved;
+  const double* h_angles;   /* copied at submit */
+} mk_shadow_request;
+
+typedef struct mk_server_stats {
+  int64_t submitted, admitted, completed, queued, in_flight;
+  int64_t items_done;       /* image rays + bisection angles finished */
+  int64_t iterations;       /* advance launches */
+  double gpu_ms;            /* summed launch time (HIP events) */
+  size_t hbm_in_use, hbm_budget;
+} mk_server_stats;
+
+int mk_server_create(const mk_server_config* cfg, mk_server** out);
+/* register a resident snapshot (not copied; keep it alive) -> *h_id */
+int mk_server_add_snapshot(mk_server* srv, const mk_snapshot* snap, int32_t* h_id);
+/* queue an image; h_out (resolution^2 doubles, pixel i*res+j) is written on
+ * completion; MK_ERR_NOMEM if its footprint can never fit the budget */
+int mk_server_submit_image(mk_server* srv, const mk_image_request* req, double* h_out, int64_t* h_id);
+/* queue a shadow bisection; h_out (n_angles doubles) written on completion */
+int mk_server_submit_shadow(mk_server* srv, const mk_shadow_request* req, double* h_out, int64_t* h_id);
+/* completed requests since the last poll: ids and submit-to-done latency */
+int mk_server_poll(mk_server* srv, int64_t* h_ids, double* h_latency_ms, int32_t cap, int32_t* h_n);
+/* block until request id (or all, id < 0) completes; MK_ERR_TIMEOUT */
+int mk_server_wait(mk_server* srv, int64_t id, double timeout_s);
+int mk_server_get_stats(mk_server* srv, mk_server_stats* h_stats);
+int mk_server_destroy(mk_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,39 @@ class mk_render_params(C.Structure):
     ]
 
 
+class mk_server_config(C.Structure):
+    _fields_ = [
+        ("precision", i32), ("lanes", i32), ("steps_per_launch", i32), ("max_requests", i32),
+        ("hbm_budget", C.c_size_t), ("use_graph", i32), ("flags", i32),
+    ]
+
+
+class mk_image_request(C.Structure):
+    _fields_ = [
+        ("snapshot", i32), ("resolution", i32), ("max_nsteps", i32), ("reserved", i32),
+        ("bhspin", dbl), ("inclination", dbl), ("distance", dbl), ("fov", dbl),
+        ("observing_frequency", dbl), ("mass_scale", dbl), ("M_bh", dbl), ("r_high", dbl),
+        ("fluid_gamma", dbl),
+    ]
+
+
+class mk_shadow_request(C.Structure):
+    _fields_ = [
+        ("bhspin", dbl), ("inclination", dbl), ("distance", dbl), ("error_allowed", dbl),
+        ("n_angles", i32), ("max_steps", i32), ("max_it", i32), ("reserved", i32),
+        ("h_angles", vp),
+    ]
+
+
+class mk_server_stats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
+        ("queued", C.c_int64), ("in_flight", C.c_int64), ("items_done", C.c_int64),
+        ("iterations", C.c_int64), ("gpu_ms", dbl), ("hbm_in_use", C.c_size_t),
+        ("hbm_budget", C.c_size_t),
+    ]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPES)
 _SIGS = {
     "mk_last_error": [],
@@ -81,6 +114,14 @@ _SIGS = {
     "mk_sample_prims": [vp, vp, lng, vp, vp],
     "mk_sample_scalars": [vp, vp, lng, dbl, dbl, vp, vp],
     "mk_render": [vp, C.POINTER(mk_render_params), vp, vp, vp],
+    "mk_server_create": [C.POINTER(mk_server_config), C.POINTER(vp)],
+    "mk_server_add_snapshot": [vp, vp, C.POINTER(i32)],
+    "mk_server_submit_image": [vp, C.POINTER(mk_image_request), vp, C.POINTER(C.c_int64)],
+    "mk_server_submit_shadow": [vp, C.POINTER(mk_shadow_request), vp, C.POINTER(C.c_int64)],
+    "mk_server_poll": [vp, vp, vp, i32, C.POINTER(i32)],
+    "mk_server_wait": [vp, C.c_int64, dbl],
+    "mk_server_get_stats": [vp, C.POINTER(mk_server_stats)],
+    "mk_server_destroy": [vp],
 }
 _RESTYPES = {"mk_last_error": C.c_char_p}
 

@@ -1,0 +1,788 @@
+// mk_serve.hip -- continuous batching of image and shadow requests.
+//
+// A worker thread owns one non-blocking stream and iterates:
+//   admit   queued requests whose output buffers fit the HBM budget: write
+//           their descriptors and a segment (request id, item count) to the
+//           device work ring;
+//   launch  k_serve: every lane owns a ray slot, runs up to M RK4 steps, and
+//           when its image ray or bisection angle finishes it takes the next
+//           work item of any admitted request from the ring (bounded atomic
+//           retries, so every lane exits);
+//   collect per-request done counters; finished outputs are copied to the
+//           caller's buffer and their arena blocks freed.
+// The launch and the counter copy are one hipGraph replayed per iteration.
+// Ray state stays in the slots between launches, so a request's rays see the
+// same arithmetic alone or mixed with others; per-ray results equal
+// mk_render / mk_shadow_bisect.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <set>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "mk_device.h"
+#include "mk_internal.h"
+#include "mk_render.h"
+#include "mk_snapshot.h"
+
+namespace mk {
+
+enum : int { kKindImage = 1, kKindShadow = 2 };
+
+// Device descriptor of one admitted request.
+struct ReqDev {
+  int kind, n_items, res, N, iters, pad;
+  double a, ci, si, dist, lo, step, rEH, div, inv_div, tol;
+  ImageParams ip;
+  SnapView sv;
+  double* out;
+  const double* cang;  // shadow: cos / sin of the angles
+  const double* sang;
+};
+
+struct Seg {
+  int rid, count, next, pad;
+};
+
+// Slot arrays, structure of arrays over L lanes.  f[] holds, per lane:
+//   0..7 state, 8 dt, 9 r, 10 I | inner, 11 trans | outer, 12 best,
+//   13 r_prev, 14 r_at.   i[]: 0 step, 1 accepted | iteration, 2 besti,
+//   3 rid, 4 item.
+constexpr int kSlotF = 15, kSlotI = 5;
+
+struct ServeArgs {
+  const ReqDev* req;
+  int* done;
+  Seg* segs;
+  int* seg_head;
+  const int* seg_tail;
+  int seg_cap;
+  double* f;
+  int* i;
+  int L, M;
+};
+
+__device__ __forceinline__ bool pop_item(const ServeArgs& A, int& rid, int& item) {
+  const int tail = *A.seg_tail;
+  for (int guard = 0; guard < 64; ++guard) {
+    const int h = __hip_atomic_load(A.seg_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (h >= tail) return false;
+    Seg* sg = A.segs + (h % A.seg_cap);
+    const int k = atomicAdd(&sg->next, 1);
+    if (k < sg->count) {
+      rid = sg->rid;
+      item = k;
+      return true;
+    }
+    atomicCAS(A.seg_head, h, h + 1);
+  }
+  return false;
+}
+
+struct Lane {
+  double s[8], dt, r, f10, f11, best, r_prev, r_at;
+  int k, n, besti;
+};
+
+__device__ __forceinline__ void shadow_begin(const ReqDev& q, int item, Lane& L) {
+  const double mid = (L.f11 - L.f10) / 2 + L.f10;  // (outer - inner)/2 + inner
+  double sx[4], sv4[4];
+  camera_ray(q.cang[item] * mid, q.sang[item] * mid, q.ci, q.si, q.dist, sx, sv4);
+  nullify(sx, sv4, q.a, L.s);
+  L.r = radius_cal(L.s[1], L.s[2], L.s[3], q.a);
+  L.dt = step_rule(L.r, q.rEH, 40.0, 1e-2);
+  L.k = 0;
+  L.best = -__builtin_inf();
+  L.besti = -1;
+  L.r_prev = __builtin_nan("");
+  L.r_at = __builtin_nan("");
+}
+
+// Starts work item `item` of request q in lane state L.  Returns true when
+// the item is already finished (a bisection with no halvings keeps the
+// initial inner radius 0.5).
+__device__ __forceinline__ bool item_begin(const ReqDev& q, int item, Lane& L) {
+  if (q.kind == kKindImage) {
+    const int i = item / q.res, j = item % q.res;
+    double sx[4], sv4[4];
+    camera_ray(grid_coord(q.lo, q.step, i), grid_coord(q.lo, q.step, j), q.ci, q.si, q.dist, sx, sv4);
+    nullify(sx, sv4, q.a, L.s);
+    L.r = radius_cal(L.s[1], L.s[2], L.s[3], q.a);
+    L.dt = step_rule(L.r, q.rEH, q.div, q.tol);
+    L.f10 = 0.0;  // I
+    L.f11 = 1.0;  // transmission
+    L.k = 0;
+    L.n = 0;
+    return false;
+  }
+  L.f10 = 0.5;   // inner
+  L.f11 = 10.0;  // outer
+  L.n = 0;       // bisection iteration
+  if (q.iters == 0) return true;
+  shadow_begin(q, item, L);
+  return false;
+}
+
+// One step of an image ray (the body of the fused render loop).  Returns
+// true when the ray is finished.
+template <bool MIXED>
+__device__ __forceinline__ bool image_step(const ReqDev& q, Lane& L) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  if (L.k >= q.N || L.dt == 0.0) return true;
+  double nw[8];
+  double rn, dtn;
+  if constexpr (!MIXED) {
+    rk4_step<double>(L.s, L.dt, q.a, nw);
+    rn = radius_cal(nw[1], nw[2], nw[3], q.a);
+    dtn = step_rule(rn, q.rEH, q.div, q.tol);
+  } else {
+    rk4_step_mixed(L.s, L.dt, (float)q.a, nw);
+    rn = radius_fast(nw[1], nw[2], nw[3], q.a);
+    dtn = step_rule_fast(rn, q.rEH, q.inv_div, q.div, q.tol);
+  }
+  if (dtn == 0.0) return true;
+  if (L.k + 1 < q.N) {
+    const Stencil st = locate(q.sv, q.sv.meta, q.sv.grid, nw[1], nw[2], nw[3]);
+    if (st.mb >= 0) {
+      Corners c;
+      fetch(q.sv, st, c);
+      T P[8];
+      interp<T>(st, c, P);
+      if (P[0] != T(0)) {
+        T j, al;
+        emission_from_prims<T>(q.ip, P, nw, q.a, j, al);
+        const double cc = -L.dt * q.ip.L_unit;
+        L.f10 += L.f11 * (cc * (double)j);
+        L.f11 *= (1.0 - cc * (double)al);
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) L.s[c] = nw[c];
+  L.dt = dtn;
+  L.r = rn;
+  ++L.k;
+  return L.k >= q.N;
+}
+
+// One row of the bisection's integration (the body of walk_ray), and the
+// bisection update when an integration ends.  Returns true when the angle's
+// last iteration is done (its radius is in L.f10).
+__device__ __forceinline__ bool shadow_step(const ReqDev& q, int item, Lane& L) {
+  const double r_here = L.r;
+  bool frozen = (L.dt == 0.0);
+  double nw[8], rn = 0.0, dtn = 0.0;
+  if (!frozen) {
+    rk4_step<double>(L.s, L.dt, q.a, nw);
+    rn = radius_cal(nw[1], nw[2], nw[3], q.a);
+    dtn = step_rule(rn, q.rEH, 40.0, 1e-2);
+    frozen = (dtn == 0.0);
+  }
+  const double row_dt = frozen ? 0.0 : L.dt;
+  if (row_dt > L.best) {
+    L.best = row_dt;
+    L.besti = L.k;
+    L.r_at = L.r_prev;
+  }
+  bool end = frozen;
+  if (!end) {
+    L.r_prev = r_here;
+    if (L.k == q.N - 1) {
+      end = true;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) L.s[c] = nw[c];
+      L.dt = dtn;
+      L.r = rn;
+      ++L.k;
+    }
+  }
+  if (!end) return false;
+  const double sel = (L.besti >= 1) ? L.r_at : L.r;
+  const double mid = (L.f11 - L.f10) / 2 + L.f10;
+  if (sel < 100.0) L.f10 = mid;
+  else L.f11 = mid;
+  if (++L.n < q.iters) {
+    shadow_begin(q, item, L);
+    return false;
+  }
+  return true;
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= A.L) return;
+  const long L_ = A.L;
+  int rid = A.i[3 * L_ + l], item = A.i[4 * L_ + l];
+  Lane L{};
+  if (rid >= 0) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) L.s[c] = A.f[c * L_ + l];
+    L.dt = A.f[8 * L_ + l];
+    L.r = A.f[9 * L_ + l];
+    L.f10 = A.f[10 * L_ + l];
+    L.f11 = A.f[11 * L_ + l];
+    L.best = A.f[12 * L_ + l];
+    L.r_prev = A.f[13 * L_ + l];
+    L.r_at = A.f[14 * L_ + l];
+    L.k = A.i[l];
+    L.n = A.i[L_ + l];
+    L.besti = A.i[2 * L_ + l];
+  }
+  int budget = A.M;
+  while (budget > 0) {
+    bool fin;
+    if (rid < 0) {
+      if (!pop_item(A, rid, item)) break;
+      fin = item_begin(A.req[rid], item, L);
+    } else {
+      const ReqDev& q = A.req[rid];
+      if (q.kind == kKindImage) fin = image_step<MIXED>(q, L);
+      else fin = shadow_step(q, item, L);
+      --budget;
+    }
+    if (fin) {
+      const ReqDev& q = A.req[rid];
+      q.out[item] = L.f10;
+      atomicAdd(&A.done[rid], 1);
+      rid = -1;
+    }
+  }
+  A.i[3 * L_ + l] = rid;
+  A.i[4 * L_ + l] = item;
+  if (rid >= 0) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) A.f[c * L_ + l] = L.s[c];
+    A.f[8 * L_ + l] = L.dt;
+    A.f[9 * L_ + l] = L.r;
+    A.f[10 * L_ + l] = L.f10;
+    A.f[11 * L_ + l] = L.f11;
+    A.f[12 * L_ + l] = L.best;
+    A.f[13 * L_ + l] = L.r_prev;
+    A.f[14 * L_ + l] = L.r_at;
+    A.i[l] = L.k;
+    A.i[L_ + l] = L.n;
+    A.i[2 * L_ + l] = L.besti;
+  }
+}
+
+__global__ void k_fill_int(int* p, long n, int v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace mk
+
+using namespace mk;
+using Clock = std::chrono::steady_clock;
+
+struct mk_server {
+  mk_server_config cfg{};
+  hipStream_t stream = nullptr;
+  int dev = 0, L = 0, M = 0, Rmax = 0, seg_cap = 4096;
+  // device
+  ReqDev* d_req = nullptr;
+  int* d_done = nullptr;
+  Seg* d_segs = nullptr;
+  int* d_head = nullptr;
+  int* d_tail = nullptr;
+  double* d_f = nullptr;
+  int* d_i = nullptr;
+  char* d_arena = nullptr;
+  size_t arena_bytes = 0;
+  // pinned host mirrors
+  ReqDev* h_req = nullptr;
+  Seg* h_segs = nullptr;
+  int* h_done = nullptr;   // Rmax + 1 (last: head)
+  int* h_tail = nullptr;
+  int* h_zero = nullptr;
+  hipGraphExec_t graph = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // host state
+  std::vector<const mk_snapshot*> snaps;
+  std::map<size_t, size_t> free_blocks;  // offset -> size
+  size_t in_use = 0;
+  struct Pending {
+    int64_t id;
+    int kind;
+    mk_image_request img;
+    mk_shadow_request sh;
+    std::vector<double> angles;
+    double* h_out;
+    size_t bytes;
+    Clock::time_point t0;
+  };
+  struct Flight {
+    int64_t id;
+    int n_items;
+    size_t off, bytes, out_bytes;
+    double* h_out;
+    Clock::time_point t0;
+    std::vector<double> cs;  // shadow: cos, sin of the angles (copy source)
+  };
+  std::deque<Pending> waiting;
+  std::map<int, Flight> flight;
+  std::vector<int> free_rids;
+  std::vector<std::pair<int64_t, double>> unpolled;
+  std::set<int64_t> done_ids;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::thread worker;
+  std::atomic<bool> stop{false};
+  int64_t next_id = 1;
+  int tail = 0, head = 0;
+  int status = MK_OK;
+  std::string err;
+  mk_server_stats st{};
+};
+
+namespace {
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+bool arena_alloc(mk_server* s, size_t bytes, size_t* off) {
+  for (auto it = s->free_blocks.begin(); it != s->free_blocks.end(); ++it) {
+    if (it->second >= bytes) {
+      *off = it->first;
+      const size_t rest = it->second - bytes;
+      const size_t o = it->first;
+      s->free_blocks.erase(it);
+      if (rest) s->free_blocks[o + bytes] = rest;
+      s->in_use += bytes;
+      return true;
+    }
+  }
+  return false;
+}
+
+void arena_free(mk_server* s, size_t off, size_t bytes) {
+  s->in_use -= bytes;
+  auto it = s->free_blocks.emplace(off, bytes).first;
+  auto nx = std::next(it);
+  if (nx != s->free_blocks.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    s->free_blocks.erase(nx);
+  }
+  if (it != s->free_blocks.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      s->free_blocks.erase(it);
+    }
+  }
+}
+
+#define SRV_HIP(call)                                                      \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess) {                                                \
+      s->status = MK_ERR_HIP;                                              \
+      s->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+      return false;                                                        \
+    }                                                                      \
+  } while (0)
+
+// Moves waiting requests into flight while they fit (caller holds mu).
+bool admit(mk_server* s) {
+  while (!s->waiting.empty() && !s->free_rids.empty() && s->tail - s->head < s->seg_cap) {
+    mk_server::Pending& p = s->waiting.front();
+    size_t off;
+    if (!arena_alloc(s, p.bytes, &off)) break;  // back-pressure: wait for memory
+    const int rid = s->free_rids.back();
+    s->free_rids.pop_back();
+    ReqDev& q = s->h_req[rid];
+    std::memset(&q, 0, sizeof q);
+    mk_server::Flight f{p.id, 0, off, p.bytes, 0, p.h_out, p.t0, {}};
+    if (p.kind == kKindImage) {
+      mk_render_params rpar{};
+      const mk_image_request& r = p.img;
+      rpar.bhspin = r.bhspin;
+      rpar.inclination = r.inclination;
+      rpar.distance = r.distance;
+      rpar.fov = r.fov;
+      rpar.mass_scale = r.mass_scale;
+      rpar.M_bh = r.M_bh;
+      rpar.r_high = r.r_high;
+      rpar.fluid_gamma = r.fluid_gamma;
+      rpar.observing_frequency = r.observing_frequency;
+      rpar.div = 40.0;
+      rpar.tol = 1e-4;
+      rpar.fallback_pitch = M_PI / 3.0;
+      rpar.sigma_cut = 100.0;
+      rpar.resolution = r.resolution;
+      rpar.max_nsteps = r.max_nsteps;
+      rpar.precision = s->cfg.precision;
+      RenderDev rd;
+      fill_render_dev(&rpar, &rd);
+      q.kind = kKindImage;
+      q.n_items = r.resolution * r.resolution;
+      q.res = r.resolution;
+      q.N = r.max_nsteps;
+      q.a = rd.a; q.ci = rd.ci; q.si = rd.si; q.dist = rd.dist; q.lo = rd.lo; q.step = rd.step;
+      q.rEH = rd.rEH; q.div = rd.div; q.inv_div = rd.inv_div; q.tol = rd.tol;
+      q.ip = rd.ip;
+      q.sv = s->snaps[r.snapshot]->view();
+      q.out = (double*)(s->d_arena + off);
+    } else {
+      const mk_shadow_request& r = p.sh;
+      const int n = r.n_angles;
+      int iters = 0;
+      double inner = 0.5, outer = 10.0;
+      while (outer - inner > r.error_allowed && iters < r.max_it) {
+        inner = (outer - inner) / 2 + inner;
+        ++iters;
+      }
+      const double inc = r.inclination * M_PI / 180;
+      q.kind = kKindShadow;
+      q.n_items = n;
+      q.N = r.max_steps;
+      q.iters = iters;
+      q.a = r.bhspin;
+      q.ci = std::cos(inc);
+      q.si = std::sin(inc);
+      q.dist = r.distance;
+      q.rEH = 1.0 + std::sqrt(1.0 - r.bhspin * r.bhspin);
+      q.out = (double*)(s->d_arena + off);
+      f.cs.resize(2 * (size_t)n);
+      for (int k = 0; k < n; ++k) {
+        f.cs[k] = std::cos(p.angles[k]);
+        f.cs[n + k] = std::sin(p.angles[k]);
+      }
+      q.cang = (double*)(s->d_arena + off) + n;
+      q.sang = q.cang + n;
+    }
+    f.n_items = q.n_items;
+    f.out_bytes = (size_t)q.n_items * sizeof(double);
+    SRV_HIP(hipMemcpyAsync(s->d_req + rid, &q, sizeof q, hipMemcpyHostToDevice, s->stream));
+    SRV_HIP(hipMemcpyAsync(s->d_done + rid, s->h_zero, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    Seg& sg = s->h_segs[s->tail % s->seg_cap];
+    sg.rid = rid;
+    sg.count = q.n_items;
+    sg.next = 0;
+    sg.pad = 0;
+    SRV_HIP(hipMemcpyAsync(s->d_segs + (s->tail % s->seg_cap), &sg, sizeof sg, hipMemcpyHostToDevice, s->stream));
+    ++s->tail;
+    s->flight[rid] = std::move(f);
+    if (q.kind == kKindShadow) {
+      // the Flight entry owns the host table until the request completes
+      const std::vector<double>& cs = s->flight[rid].cs;
+      SRV_HIP(hipMemcpyAsync((void*)q.cang, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice,
+                             s->stream));
+    }
+    s->waiting.pop_front();
+    ++s->st.admitted;
+  }
+  *s->h_tail = s->tail;
+  SRV_HIP(hipMemcpyAsync(s->d_tail, s->h_tail, sizeof(int), hipMemcpyHostToDevice, s->stream));
+  return true;
+}
+
+ServeArgs make_args(mk_server* s) {
+  ServeArgs A;
+  A.req = s->d_req;
+  A.done = s->d_done;
+  A.segs = s->d_segs;
+  A.seg_head = s->d_head;
+  A.seg_tail = s->d_tail;
+  A.seg_cap = s->seg_cap;
+  A.f = s->d_f;
+  A.i = s->d_i;
+  A.L = s->L;
+  A.M = s->M;
+  return A;
+}
+
+bool enqueue_iteration(mk_server* s) {
+  const ServeArgs A = make_args(s);
+  const unsigned blocks = (unsigned)((s->L + 255) / 256);
+  if (s->cfg.precision == 0) k_serve<false><<<blocks, 256, 0, s->stream>>>(A);
+  else k_serve<true><<<blocks, 256, 0, s->stream>>>(A);
+  SRV_HIP(hipGetLastError());
+  SRV_HIP(hipMemcpyAsync(s->h_done, s->d_done, s->Rmax * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  SRV_HIP(hipMemcpyAsync(s->h_done + s->Rmax, s->d_head, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  return true;
+}
+
+bool build_graph(mk_server* s) {
+  hipGraph_t g;
+  SRV_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  const bool ok = enqueue_iteration(s);
+  hipError_t e = hipStreamEndCapture(s->stream, &g);
+  if (!ok) return false;
+  SRV_HIP(e);
+  SRV_HIP(hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0));
+  SRV_HIP(hipGraphDestroy(g));
+  return true;
+}
+
+bool iterate(mk_server* s) {
+  SRV_HIP(hipEventRecord(s->ev0, s->stream));
+  if (s->graph) {
+    SRV_HIP(hipGraphLaunch(s->graph, s->stream));
+  } else if (!enqueue_iteration(s)) {
+    return false;
+  }
+  SRV_HIP(hipEventRecord(s->ev1, s->stream));
+  SRV_HIP(hipEventSynchronize(s->ev1));
+  float ms = 0;
+  SRV_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  s->st.gpu_ms += ms;
+  ++s->st.iterations;
+  return true;
+}
+
+// Finished requests: copy out, free, report (caller holds mu).
+bool collect(mk_server* s) {
+  s->head = s->h_done[s->Rmax];
+  std::vector<int> fin;
+  for (auto& kv : s->flight)
+    if (s->h_done[kv.first] >= kv.second.n_items) fin.push_back(kv.first);
+  for (int rid : fin) {
+    const mk_server::Flight& f = s->flight[rid];
+    SRV_HIP(hipMemcpyAsync(f.h_out, s->d_arena + f.off, f.out_bytes, hipMemcpyDeviceToHost, s->stream));
+    SRV_HIP(hipStreamSynchronize(s->stream));
+    const int64_t id = f.id;
+    const int n_items = f.n_items;
+    const double ms = std::chrono::duration<double, std::milli>(Clock::now() - f.t0).count();
+    arena_free(s, f.off, f.bytes);
+    s->flight.erase(rid);
+    s->free_rids.push_back(rid);
+    s->unpolled.emplace_back(id, ms);
+    s->done_ids.insert(id);
+    ++s->st.completed;
+    s->st.items_done += n_items;
+  }
+  if (!fin.empty()) s->cv_done.notify_all();
+  return true;
+}
+
+void worker_main(mk_server* s) {
+  if (hipSetDevice(s->dev) != hipSuccess) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->status = MK_ERR_HIP;
+    s->err = "worker: hipSetDevice failed";
+    s->cv_done.notify_all();
+    return;
+  }
+  while (!s->stop.load()) {
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      if (!admit(s)) break;
+      if (s->flight.empty()) {
+        s->cv_work.wait_for(lk, std::chrono::milliseconds(20));
+        continue;
+      }
+    }
+    if (!iterate(s)) break;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!collect(s)) break;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->cv_done.notify_all();
+}
+
+int server_error(mk_server* s) {
+  if (s->status == MK_OK) return MK_OK;
+  set_error("mk_server: %s", s->err.c_str());
+  return s->status;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mk_server_create(const mk_server_config* cfg, mk_server** out) {
+  MK_REQUIRE(cfg && out, "mk_server_create: null argument");
+  MK_REQUIRE(cfg->precision == 0 || cfg->precision == 1, "mk_server_create: precision must be 0 or 1");
+  *out = nullptr;
+  mk_server* s = new mk_server();
+  s->cfg = *cfg;
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  s->dev = dev;
+  s->L = cfg->lanes > 0 ? cfg->lanes : std::max(cus, 1) * 4 * 2 * 64;
+  s->M = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : 256;
+  s->Rmax = cfg->max_requests > 0 ? cfg->max_requests : 1024;
+  s->arena_bytes = cfg->hbm_budget > 0 ? cfg->hbm_budget : ((size_t)4 << 30);
+  const long L = s->L;
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&s->d_req, s->Rmax * sizeof(ReqDev));
+  if (e == hipSuccess) e = hipMalloc(&s->d_done, s->Rmax * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&s->d_segs, s->seg_cap * sizeof(Seg));
+  if (e == hipSuccess) e = hipMalloc(&s->d_head, 2 * sizeof(int));
+  if (e == hipSuccess) s->d_tail = s->d_head + 1;
+  if (e == hipSuccess) e = hipMalloc(&s->d_f, (size_t)kSlotF * L * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&s->d_i, (size_t)kSlotI * L * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&s->d_arena, s->arena_bytes);
+  if (e == hipSuccess) e = hipHostMalloc(&s->h_req, s->Rmax * sizeof(ReqDev), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&s->h_segs, s->seg_cap * sizeof(Seg), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&s->h_done, (s->Rmax + 1) * sizeof(int), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&s->h_tail, 2 * sizeof(int), hipHostMallocDefault);
+  if (e == hipSuccess) {
+    s->h_zero = s->h_tail + 1;
+    *s->h_zero = 0;
+    *s->h_tail = 0;
+  }
+  if (e == hipSuccess) e = hipEventCreate(&s->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+  if (e == hipSuccess) {
+    // slots empty (rid = -1), counters zero: fill kernels, no memset nodes
+    k_fill_int<<<grid_for(kSlotI * L, 256), 256, 0, s->stream>>>(s->d_i, kSlotI * L, -1);
+    k_fill_int<<<grid_for(s->Rmax, 256), 256, 0, s->stream>>>(s->d_done, s->Rmax, 0);
+    k_fill_int<<<1, 64, 0, s->stream>>>(s->d_head, 2, 0);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) {
+    set_error("mk_server_create: %s", hipGetErrorString(e));
+    mk_server_destroy(s);
+    return e == hipErrorOutOfMemory ? MK_ERR_NOMEM : MK_ERR_HIP;
+  }
+  s->free_blocks[0] = s->arena_bytes;
+  for (int r = s->Rmax - 1; r >= 0; --r) s->free_rids.push_back(r);
+  s->st.hbm_budget = s->arena_bytes;
+  if (cfg->use_graph && !build_graph(s)) {
+    set_error("mk_server_create: graph capture failed: %s", s->err.c_str());
+    mk_server_destroy(s);
+    return MK_ERR_HIP;
+  }
+  s->worker = std::thread(worker_main, s);
+  *out = s;
+  return MK_OK;
+}
+
+int mk_server_add_snapshot(mk_server* s, const mk_snapshot* snap, int32_t* id) {
+  MK_REQUIRE(s && snap && id, "mk_server_add_snapshot: null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->snaps.push_back(snap);
+  *id = (int32_t)s->snaps.size() - 1;
+  return MK_OK;
+}
+
+int mk_server_submit_image(mk_server* s, const mk_image_request* r, double* h_out, int64_t* id) {
+  MK_REQUIRE(s && r && h_out && id, "mk_server_submit_image: null argument");
+  MK_REQUIRE(r->resolution > 0 && r->resolution <= 8192 && r->max_nsteps >= 1, "mk_server_submit_image: bad size");
+  MK_REQUIRE(std::fabs(r->bhspin) <= 1.0 && r->observing_frequency > 0 && r->M_bh > 0 && r->mass_scale > 0,
+             "mk_server_submit_image: bad physical parameters");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (int rc = server_error(s)) return rc;
+  MK_REQUIRE(r->snapshot >= 0 && r->snapshot < (int)s->snaps.size(), "mk_server_submit_image: unknown snapshot");
+  mk_server::Pending p{};
+  p.id = s->next_id++;
+  p.kind = kKindImage;
+  p.img = *r;
+  p.h_out = h_out;
+  p.bytes = align_up((size_t)r->resolution * r->resolution * sizeof(double));
+  if (p.bytes > s->arena_bytes) {
+    set_error("mk_server_submit_image: %zu bytes of output exceed the %zu-byte budget", p.bytes, s->arena_bytes);
+    return MK_ERR_NOMEM;
+  }
+  p.t0 = Clock::now();
+  s->waiting.push_back(std::move(p));
+  ++s->st.submitted;
+  *id = s->waiting.back().id;
+  s->cv_work.notify_one();
+  return MK_OK;
+}
+
+int mk_server_submit_shadow(mk_server* s, const mk_shadow_request* r, double* h_out, int64_t* id) {
+  MK_REQUIRE(s && r && h_out && id && r->h_angles, "mk_server_submit_shadow: null argument");
+  MK_REQUIRE(r->n_angles > 0 && r->max_steps >= 1 && std::fabs(r->bhspin) <= 1.0, "mk_server_submit_shadow: bad size");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (int rc = server_error(s)) return rc;
+  mk_server::Pending p{};
+  p.id = s->next_id++;
+  p.kind = kKindShadow;
+  p.sh = *r;
+  p.angles.assign(r->h_angles, r->h_angles + r->n_angles);
+  p.sh.h_angles = nullptr;
+  p.h_out = h_out;
+  p.bytes = align_up((size_t)3 * r->n_angles * sizeof(double));
+  if (p.bytes > s->arena_bytes) {
+    set_error("mk_server_submit_shadow: output exceeds the budget");
+    return MK_ERR_NOMEM;
+  }
+  p.t0 = Clock::now();
+  s->waiting.push_back(std::move(p));
+  ++s->st.submitted;
+  *id = s->waiting.back().id;
+  s->cv_work.notify_one();
+  return MK_OK;
+}
+
+int mk_server_poll(mk_server* s, int64_t* ids, double* lat, int32_t cap, int32_t* n) {
+  MK_REQUIRE(s && n && (cap == 0 || (ids && lat)), "mk_server_poll: null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  const int m = std::min<int>(cap, (int)s->unpolled.size());
+  for (int k = 0; k < m; ++k) {
+    ids[k] = s->unpolled[k].first;
+    lat[k] = s->unpolled[k].second;
+  }
+  s->unpolled.erase(s->unpolled.begin(), s->unpolled.begin() + m);
+  *n = m;
+  return server_error(s);
+}
+
+int mk_server_wait(mk_server* s, int64_t id, double timeout_s) {
+  MK_REQUIRE(s, "mk_server_wait: null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto ready = [&] {
+    if (s->status != MK_OK) return true;
+    if (id >= 0) return s->done_ids.count(id) > 0;
+    return s->waiting.empty() && s->flight.empty();
+  };
+  if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
+    set_error("mk_server_wait: timed out after %.3f s", timeout_s);
+    return MK_ERR_TIMEOUT;
+  }
+  return server_error(s);
+}
+
+int mk_server_get_stats(mk_server* s, mk_server_stats* out) {
+  MK_REQUIRE(s && out, "mk_server_get_stats: null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  *out = s->st;
+  out->queued = (int64_t)s->waiting.size();
+  out->in_flight = (int64_t)s->flight.size();
+  out->hbm_in_use = s->in_use;
+  return server_error(s);
+}
+
+int mk_server_destroy(mk_server* s) {
+  if (!s) return MK_OK;
+  s->stop.store(true);
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  (void)hipFree(s->d_req);
+  (void)hipFree(s->d_done);
+  (void)hipFree(s->d_segs);
+  (void)hipFree(s->d_head);
+  (void)hipFree(s->d_f);
+  (void)hipFree(s->d_i);
+  (void)hipFree(s->d_arena);
+  if (s->h_req) (void)hipHostFree(s->h_req);
+  if (s->h_segs) (void)hipHostFree(s->h_segs);
+  if (s->h_done) (void)hipHostFree(s->h_done);
+  if (s->h_tail) (void)hipHostFree(s->h_tail);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return MK_OK;
+}
+
+}  // extern "C"

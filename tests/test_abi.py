@@ -70,3 +70,28 @@ def test_mahakala_alias_exposes_reference_surface():
     for name in ("metric", "imetric", "radius_cal", "radius_EH", "RK4_gen", "rhs", "initial_condition",
                  "get_initial_grid", "get_camera_pixel", "select_photons_integrator"):
         assert callable(getattr(mg, name))
+
+
+def test_server_fails_loudly_without_a_device():
+    L = _lib.lib()
+    if L.mk_device_info(None, 0, None, None) == 0:
+        pytest.skip("a GPU is visible")
+    from mahakala_amd.serving import Server
+    with pytest.raises(_lib.MkError):
+        Server("mixed")
+    with pytest.raises(ValueError):
+        Server("fp16")
+    cfg = _lib.mk_server_config(3, 0, 0, 0, 0, 0, 0)
+    h = _lib.vp()
+    assert L.mk_server_create(C.byref(cfg), C.byref(h)) == -1
+
+
+def test_server_structs_match_header():
+    text = open(_lib.HEADER_PATH).read()
+    for name in ("mk_server_config", "mk_image_request", "mk_shadow_request", "mk_server_stats"):
+        block = text[text.index(f"typedef struct {name}"):text.index(f"}} {name};")]
+        block = re.sub(r"/\*.*?\*/", "", block, flags=re.S)
+        fields = []
+        for m in re.finditer(r"^\s*(?:const\s+)?(?:double|int32_t|int64_t|size_t)\*?\s+([\w,\s*]+);", block, re.M):
+            fields += [f.strip().lstrip("*") for f in m.group(1).split(",")]
+        assert fields == [n for n, _ in getattr(_lib, name)._fields_], name

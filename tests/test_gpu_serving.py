@@ -1,0 +1,134 @@
+"""GPU tests of the request server (continuous batching of image and shadow
+requests; images.py:30-144 and geodesics.py:405-435 of the reference).
+
+* a request's result is bit-identical alone and mixed with other requests
+  (different resolution, spin, inclination, frequency, snapshot, kind), with
+  slots reused across launches;
+* served images equal mk_render's and served bisections equal
+  mk_shadow_bisect's (same arithmetic per ray);
+* admission respects the HBM budget (back-pressure, never-fitting requests
+  rejected) and wait() times out.
+"""
+import numpy as np
+import pytest
+
+from mahakala_amd import _lib, images
+from mahakala_amd.geodesics import find_shadow_bisection_angles
+from mahakala_amd.grmhd import synthetic as syn
+from mahakala_amd.grmhd.athenak import AthenakFluidModel
+from mahakala_amd.serving import Server
+from oracle import c_oracle
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+MSUN = 1.989e33
+
+# (resolution, spin, inclination, frequency, snapshot index)
+MIX = [(48, 0.5, 30.0, 230e9, 0), (32, -0.7, 75.0, 86e9, 1), (64, 0.9, 10.0, 345e9, 0), (40, 0.0, 50.0, 230e9, 1)]
+
+
+@pytest.fixture(scope="module")
+def models():
+    torus = AthenakFluidModel.from_datasets(syn.config2_snapshot(0.5, seed=0), 0.5, fluid_gamma=13. / 9)
+    small = AthenakFluidModel.from_datasets(helpers.small_two_level(n=4, seed=3), 0.5, fluid_gamma=13. / 9)
+    return torus, small
+
+
+def _img_kw(res, spin, inc, nu):
+    return dict(bhspin=spin, inclination=inc, resolution=res, observing_frequency=nu, fov=20, mass_scale=1e26,
+                max_nsteps=4000)
+
+
+def _render(model, res, spin, inc, nu, precision):
+    p = images.render_params(spin, inc, 1000.0, 1e26, 6.2e9 * MSUN, 40, nu, 20, res, 4000, 13. / 9, precision)
+    return images.render(model.device_snapshot, p)
+
+
+@pytest.mark.parametrize("precision", ["plain64", "mixed"])
+def test_served_images_equal_render_and_do_not_depend_on_the_mix(models, precision):
+    print(f"[serving] precision={precision}", flush=True)
+    alone = {}
+    # few lanes and short launches: rays of many requests share waves and
+    # each slot is reused many times
+    with Server(precision, lanes=2048, steps_per_launch=64) as srv:
+        sids = [srv.add_snapshot(m) for m in models]
+        for k, (res, spin, inc, nu, s) in enumerate(MIX):
+            rid = srv.submit_image(sids[s], **_img_kw(res, spin, inc, nu))
+            alone[k] = srv.result(rid, timeout=120)
+    with Server(precision, lanes=2048, steps_per_launch=64) as srv:
+        sids = [srv.add_snapshot(m) for m in models]
+        ang = np.linspace(0, 2 * np.pi, 24, endpoint=False)
+        sh = srv.submit_shadow(0.3, 40.0, ang, max_steps=1500)
+        rids = [srv.submit_image(sids[s], **_img_kw(res, spin, inc, nu)) for res, spin, inc, nu, s in MIX[::-1]]
+        mixed = {len(MIX) - 1 - i: srv.result(r, timeout=120) for i, r in enumerate(rids)}
+        srv.result(sh, timeout=120)
+        st = srv.stats()
+    assert st["completed"] == len(MIX) + 1 and st["in_flight"] == 0 and st["hbm_in_use"] == 0
+    for k, (res, spin, inc, nu, s) in enumerate(MIX):
+        assert np.array_equal(alone[k], mixed[k]), k
+        # same per-ray arithmetic as mk_render; the two kernels are compiled
+        # separately, so FMA contraction may differ by an ulp per operation
+        ref = _render(models[s], res, spin, inc, nu, precision)
+        assert s == 1 or np.count_nonzero(ref) > res
+        live = ref > 1e-6 * max(ref.max(), 1e-300)
+        rel = np.abs(alone[k][live] - ref[live]) / ref[live]
+        flux = abs(alone[k].sum() - ref.sum()) / max(ref.sum(), 1e-300)
+        print(f"[serving] {precision} request {k}: max rel {rel.max() if rel.size else 0:.2e} flux {flux:.2e}")
+        if precision == "plain64":
+            assert rel.size == 0 or rel.max() < 1e-10
+        else:
+            # FP32 rounding may differ by contraction between the two kernels.
+            # On the torus: flux within 1e-4 of mk_render's mixed image and
+            # within the 1e-3 precision bar of plain FP64.  The two-level toy
+            # field (intensities near 1e163, not physical) amplifies FP32
+            # rounding; there only alone == mixed is asserted.
+            if s == 0:
+                assert flux < 1e-4
+                ref64 = _render(models[s], res, spin, inc, nu, "plain64")
+                assert abs(alone[k].sum() - ref64.sum()) <= 1e-3 * ref64.sum()
+
+
+def test_served_shadow_equals_bisection_kernel():
+    ang = np.linspace(0, 2 * np.pi, 40, endpoint=False)
+    ref_a = find_shadow_bisection_angles(0.9, 60.0, ang, max_steps=2000)
+    ref_b = find_shadow_bisection_angles(-0.4, 20.0, ang[::3], max_steps=1500, error_allowed=1e-2)
+    with Server("mixed", lanes=1024, steps_per_launch=128) as srv:
+        a = srv.submit_shadow(0.9, 60.0, ang, max_steps=2000)
+        b = srv.submit_shadow(-0.4, 20.0, ang[::3], max_steps=1500, error_allowed=1e-2)
+        c = srv.submit_shadow(0.1, 45.0, ang[:5], error_allowed=20.0)  # no halving: 0.5
+        np.testing.assert_array_equal(srv.result(a, 120), ref_a)
+        np.testing.assert_array_equal(srv.result(b, 120), ref_b)
+        np.testing.assert_array_equal(srv.result(c, 120), np.full(5, 0.5))
+    # and the oracle's bisection to 1e-6 M
+    np.testing.assert_allclose(ref_a, c_oracle.find_shadow_bisection_angles(0.9, 60.0, ang, 2000, 1e-3, 40),
+                               rtol=0, atol=1e-6)
+    assert np.all((ref_a > 3) & (ref_a < 7))
+
+
+def test_budget_back_pressure_and_errors(models):
+    res = 48
+    need = res * res * 8  # one output buffer (256-byte aligned already)
+    with Server("mixed", hbm_budget=need, lanes=4096) as srv:
+        sid = srv.add_snapshot(models[0])
+        ref = _render(models[0], res, 0.5, 30.0, 230e9, "mixed")
+        rids = [srv.submit_image(sid, **_img_kw(res, 0.5, 30.0, 230e9)) for _ in range(3)]
+        st = srv.stats()
+        assert st["hbm_budget"] == need and st["hbm_in_use"] <= need
+        outs = [srv.result(r, timeout=120) for r in rids]
+        for o in outs:
+            np.testing.assert_allclose(o, ref, rtol=1e-4, atol=1e-6 * ref.max())
+        np.testing.assert_array_equal(o, outs[0])
+        with pytest.raises(_lib.MkError, match="budget"):
+            srv.submit_image(sid, **_img_kw(64, 0.5, 30.0, 230e9))
+        with pytest.raises(_lib.MkError, match="unknown snapshot"):
+            srv.submit_image(7, **_img_kw(16, 0.5, 30.0, 230e9))
+        done = srv.poll()
+        assert sorted(i for i, _ in done) == sorted(rids)
+        assert all(ms > 0 for _, ms in done)
+    with Server("plain64") as srv:
+        sid = srv.add_snapshot(models[0])
+        rid = srv.submit_image(sid, **_img_kw(256, 0.5, 30.0, 230e9))
+        with pytest.raises(_lib.MkError, match="timed out"):
+            srv.wait(rid, timeout=0.0)
+        srv.wait(-1, timeout=120)
+        assert srv.stats()["completed"] == 1
