@@ -225,6 +225,9 @@ typedef struct mk_server_stats {
   int64_t iterations;       /* advance launches */
   double gpu_ms;            /* summed launch time (HIP events) */
   size_t hbm_in_use, hbm_budget;
+  int64_t lane_steps_busy;  /* lane-steps that advanced a ray or bisection */
+  int64_t lane_steps_total; /* lanes x steps_per_launch x launches */
+  int32_t lanes, steps_per_launch; /* effective configuration */
 } mk_server_stats;
 
 int mk_server_create(const mk_server_config* cfg, mk_server** out);

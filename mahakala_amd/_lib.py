@@ -66,7 +66,8 @@ class mk_server_stats(C.Structure):
         ("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
         ("queued", C.c_int64), ("in_flight", C.c_int64), ("items_done", C.c_int64),
         ("iterations", C.c_int64), ("gpu_ms", dbl), ("hbm_in_use", C.c_size_t),
-        ("hbm_budget", C.c_size_t),
+        ("hbm_budget", C.c_size_t), ("lane_steps_busy", C.c_int64), ("lane_steps_total", C.c_int64),
+        ("lanes", i32), ("steps_per_launch", i32),
     ]
 
 

@@ -41,7 +41,7 @@ enum : int { kKindImage = 1, kKindShadow = 2 };
 
 // Device descriptor of one admitted request.
 struct ReqDev {
-  int kind, n_items, res, N, iters, pad;
+  int kind, n_items, res, N, iters, tiled;
   double a, ci, si, dist, lo, step, rEH, div, inv_div, tol;
   ImageParams ip;
   SnapView sv;
@@ -50,18 +50,21 @@ struct ReqDev {
   const double* sang;
 };
 
+// A segment of the work ring: the tiles of one request (64 work items each:
+// an 8x8 pixel block, a run of 64 pixels, or 64 bisection angles).
 struct Seg {
   int rid, count, next, pad;
 };
 
+constexpr int kTile = 64;
+
 // Slot arrays, structure of arrays over L lanes.  f[] holds, per lane:
 //   0..7 state, 8 dt, 9 r, 10 I | inner, 11 trans | outer, 12 best,
-//   13 r_prev, 14 r_at.   i[]: 0 step, 1 accepted | iteration, 2 besti,
-//   3 rid, 4 item.
-constexpr int kSlotF = 15, kSlotI = 5;
+//   13 r_prev, 14 r_at.   i[]: 0 step, 1 bisection iteration, 2 besti,
+//   3 request of the wave, 4 tile of the wave, 5 lane still running.
+constexpr int kSlotF = 15, kSlotI = 6;
 
 struct ServeArgs {
-  const ReqDev* req;
   int* done;
   Seg* segs;
   int* seg_head;
@@ -69,24 +72,33 @@ struct ServeArgs {
   int seg_cap;
   double* f;
   int* i;
+  unsigned long long* busy;  // lane-steps that advanced a ray (cumulative)
   int L, M;
 };
 
-__device__ __forceinline__ bool pop_item(const ServeArgs& A, int& rid, int& item) {
-  const int tail = *A.seg_tail;
-  for (int guard = 0; guard < 64; ++guard) {
-    const int h = __hip_atomic_load(A.seg_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (h >= tail) return false;
-    Seg* sg = A.segs + (h % A.seg_cap);
-    const int k = atomicAdd(&sg->next, 1);
-    if (k < sg->count) {
-      rid = sg->rid;
-      item = k;
-      return true;
+// Wave-level pop of one tile; every lane gets the same (rid, tile), or
+// rid = -1 when the ring is drained.  The retry loop is bounded.
+__device__ __forceinline__ void pop_tile(const ServeArgs& A, int& rid, int& tile) {
+  int g_rid = -1, g_tile = 0;
+  if (__lane_id() == 0) {
+    const int tail = *A.seg_tail;
+    for (int guard = 0; guard < 64; ++guard) {
+      const int h = __hip_atomic_load(A.seg_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (h >= tail) break;
+      Seg* sg = A.segs + (h % A.seg_cap);
+      const int count = sg->count;
+      const int k = atomicAdd(&sg->next, 1);
+      if (k < count) {
+        g_rid = sg->rid;
+        g_tile = k;
+        if (k + 1 >= count) atomicCAS(A.seg_head, h, h + 1);
+        break;
+      }
+      atomicCAS(A.seg_head, h, h + 1);
     }
-    atomicCAS(A.seg_head, h, h + 1);
   }
-  return false;
+  rid = __builtin_amdgcn_readfirstlane(__shfl(g_rid, 0));
+  tile = __builtin_amdgcn_readfirstlane(__shfl(g_tile, 0));
 }
 
 struct Lane {
@@ -108,12 +120,22 @@ __device__ __forceinline__ void shadow_begin(const ReqDev& q, int item, Lane& L)
   L.r_at = __builtin_nan("");
 }
 
+// Image item -> pixel index i*res+j.  Items run over 8x8 tiles when the
+// resolution allows, so a wave's 64 items are neighbouring rays.
+__device__ __forceinline__ int pixel_of_item(const ReqDev& q, int item) {
+  if (!q.tiled) return item;
+  const int tile = item >> 6, w = item & 63, tx = q.res >> 3;
+  const int i = (tile / tx) * 8 + (w >> 3), j = (tile % tx) * 8 + (w & 7);
+  return i * q.res + j;
+}
+
 // Starts work item `item` of request q in lane state L.  Returns true when
 // the item is already finished (a bisection with no halvings keeps the
 // initial inner radius 0.5).
 __device__ __forceinline__ bool item_begin(const ReqDev& q, int item, Lane& L) {
   if (q.kind == kKindImage) {
-    const int i = item / q.res, j = item % q.res;
+    const int px = pixel_of_item(q, item);
+    const int i = px / q.res, j = px % q.res;
     double sx[4], sv4[4];
     camera_ray(grid_coord(q.lo, q.step, i), grid_coord(q.lo, q.step, j), q.ci, q.si, q.dist, sx, sv4);
     nullify(sx, sv4, q.a, L.s);
@@ -133,46 +155,78 @@ __device__ __forceinline__ bool item_begin(const ReqDev& q, int item, Lane& L) {
   return false;
 }
 
-// One step of an image ray (the body of the fused render loop).  Returns
-// true when the ray is finished.
+// A sample fetched one step ahead (its emission is added after the next
+// RK4 step, so the cell loads overlap that step's arithmetic).
+struct Pending {
+  Stencil st;
+  Corners c;
+  double s[8], dt;
+};
+
 template <bool MIXED>
-__device__ __forceinline__ bool image_step(const ReqDev& q, Lane& L) {
+__device__ __forceinline__ void consume(const ReqDev& q, const Pending& p, Lane& L) {
   using T = typename std::conditional<MIXED, float, double>::type;
-  if (L.k >= q.N || L.dt == 0.0) return true;
+  if (p.st.mb < 0) return;
+  T P[8], j, al;
+  interp<T>(p.st, p.c, P);
+  if (P[0] == T(0)) return;
+  emission_from_prims<T>(q.ip, P, p.s, q.a, j, al);
+  const double cc = -p.dt * q.ip.L_unit;
+  L.f10 += L.f11 * (cc * (double)j);
+  L.f11 *= (1.0 - cc * (double)al);
+}
+
+// One pass of the image-ray loop (the body of the fused render loop, split
+// around the single place where a pending sample is consumed).  `go` false
+// only settles the pending sample (the wave is out of steps).  Returns true
+// when the ray is finished.  Keeping one call site for consume() keeps the
+// arithmetic of every sample the same wherever a launch boundary falls.
+template <bool MIXED>
+__device__ __forceinline__ bool image_pass(const ReqDev& q, bool go, Lane& L, Pending& p, bool& pend) {
+  bool fin = false, adv = false;
   double nw[8];
-  double rn, dtn;
-  if constexpr (!MIXED) {
-    rk4_step<double>(L.s, L.dt, q.a, nw);
-    rn = radius_cal(nw[1], nw[2], nw[3], q.a);
-    dtn = step_rule(rn, q.rEH, q.div, q.tol);
-  } else {
-    rk4_step_mixed(L.s, L.dt, (float)q.a, nw);
-    rn = radius_fast(nw[1], nw[2], nw[3], q.a);
-    dtn = step_rule_fast(rn, q.rEH, q.inv_div, q.div, q.tol);
-  }
-  if (dtn == 0.0) return true;
-  if (L.k + 1 < q.N) {
-    const Stencil st = locate(q.sv, q.sv.meta, q.sv.grid, nw[1], nw[2], nw[3]);
-    if (st.mb >= 0) {
-      Corners c;
-      fetch(q.sv, st, c);
-      T P[8];
-      interp<T>(st, c, P);
-      if (P[0] != T(0)) {
-        T j, al;
-        emission_from_prims<T>(q.ip, P, nw, q.a, j, al);
-        const double cc = -L.dt * q.ip.L_unit;
-        L.f10 += L.f11 * (cc * (double)j);
-        L.f11 *= (1.0 - cc * (double)al);
-      }
+  if (go) {
+    if (L.k >= q.N || L.dt == 0.0) {
+      fin = true;
+    } else {
+      if constexpr (!MIXED) rk4_step<double>(L.s, L.dt, q.a, nw);
+      else rk4_step_mixed(L.s, L.dt, (float)q.a, nw);
+      adv = true;
     }
   }
+  if (pend) {
+    consume<MIXED>(q, p, L);
+    pend = false;
+  }
+  if (adv) {
+    double rn, dtn;
+    if constexpr (!MIXED) {
+      rn = radius_cal(nw[1], nw[2], nw[3], q.a);
+      dtn = step_rule(rn, q.rEH, q.div, q.tol);
+    } else {
+      rn = radius_fast(nw[1], nw[2], nw[3], q.a);
+      dtn = step_rule_fast(rn, q.rEH, q.inv_div, q.div, q.tol);
+    }
+    if (dtn == 0.0) {
+      fin = true;
+    } else {
+      if (L.k + 1 < q.N) {  // row N is never emitted
+        p.st = locate(q.sv, q.sv.meta, q.sv.grid, nw[1], nw[2], nw[3]);
+        fetch(q.sv, p.st, p.c);
 #pragma unroll
-  for (int c = 0; c < 8; ++c) L.s[c] = nw[c];
-  L.dt = dtn;
-  L.r = rn;
-  ++L.k;
-  return L.k >= q.N;
+        for (int c = 0; c < 8; ++c) p.s[c] = nw[c];
+        p.dt = L.dt;
+        pend = true;
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) L.s[c] = nw[c];
+      L.dt = dtn;
+      L.r = rn;
+      ++L.k;
+      fin = L.k >= q.N;  // no sample is pending then
+    }
+  }
+  return fin;
 }
 
 // One row of the bisection's integration (the body of walk_ray), and the
@@ -219,14 +273,22 @@ __device__ __forceinline__ bool shadow_step(const ReqDev& q, int item, Lane& L) 
   return true;
 }
 
+// Advance kernel.  Each wave owns one tile (64 work items of one request)
+// at a time: its lanes step their rays together, as in the render kernel,
+// so neighbouring rays share cache lines; when every lane of the tile is
+// done the wave takes the next tile of any admitted request.  A wave stops
+// after M steps and leaves its rays in the slot arrays for the next launch.
 template <bool MIXED>
-__global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= A.L) return;
+__global__ void __launch_bounds__(256, 2) k_serve(const ReqDev* __restrict__ req, ServeArgs A) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;  // L is a multiple of 256
+  const int lane = threadIdx.x & 63;
   const long L_ = A.L;
-  int rid = A.i[3 * L_ + l], item = A.i[4 * L_ + l];
+  int rid = __builtin_amdgcn_readfirstlane(A.i[3 * L_ + l]);
+  int tile = __builtin_amdgcn_readfirstlane(A.i[4 * L_ + l]);
   Lane L{};
+  bool live = false;
   if (rid >= 0) {
+    live = A.i[5 * L_ + l] != 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) L.s[c] = A.f[c * L_ + l];
     L.dt = A.f[8 * L_ + l];
@@ -240,27 +302,59 @@ __global__ void __launch_bounds__(256) k_serve(ServeArgs A) {
     L.n = A.i[L_ + l];
     L.besti = A.i[2 * L_ + l];
   }
-  int budget = A.M;
+  int budget = A.M, steps = 0;
   while (budget > 0) {
-    bool fin;
     if (rid < 0) {
-      if (!pop_item(A, rid, item)) break;
-      fin = item_begin(A.req[rid], item, L);
-    } else {
-      const ReqDev& q = A.req[rid];
-      if (q.kind == kKindImage) fin = image_step<MIXED>(q, L);
-      else fin = shadow_step(q, item, L);
-      --budget;
+      pop_tile(A, rid, tile);
+      if (rid < 0) break;  // ring drained
+      const ReqDev& q = req[rid];
+      const int item = tile * kTile + lane;
+      live = item < q.n_items;
+      if (live && item_begin(q, item, L)) {
+        q.out[item] = L.f10;
+        live = false;
+      }
     }
-    if (fin) {
-      const ReqDev& q = A.req[rid];
-      q.out[item] = L.f10;
-      atomicAdd(&A.done[rid], 1);
+    const ReqDev& q = req[rid];
+    const int item = tile * kTile + lane;
+    if (q.kind == kKindImage) {
+      Pending p;
+      bool pend = false;
+      while (true) {
+        const bool go = budget > 0 && __any(live);  // uniform
+        if (live) {
+          steps += go;
+          if (image_pass<MIXED>(q, go, L, p, pend)) {
+            q.out[pixel_of_item(q, item)] = L.f10;
+            live = false;
+          }
+        }
+        if (!go) break;  // the last pass only settled pending samples
+        --budget;
+      }
+    } else {
+      while (budget > 0 && __any(live)) {
+        if (live) {
+          ++steps;
+          if (shadow_step(q, item, L)) {
+            q.out[item] = L.f10;
+            live = false;
+          }
+        }
+        --budget;
+      }
+    }
+    if (!__any(live)) {  // tile done: count its items once
+      if (lane == 0) atomicAdd(&A.done[rid], min(kTile, q.n_items - tile * kTile));
       rid = -1;
     }
   }
+  unsigned long long w = (unsigned long long)steps;
+  for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
+  if (lane == 0) atomicAdd(A.busy, w);
   A.i[3 * L_ + l] = rid;
-  A.i[4 * L_ + l] = item;
+  A.i[4 * L_ + l] = tile;
+  A.i[5 * L_ + l] = live ? 1 : 0;
   if (rid >= 0) {
 #pragma unroll
     for (int c = 0; c < 8; ++c) A.f[c * L_ + l] = L.s[c];
@@ -304,7 +398,7 @@ struct mk_server {
   // pinned host mirrors
   ReqDev* h_req = nullptr;
   Seg* h_segs = nullptr;
-  int* h_done = nullptr;   // Rmax + 1 (last: head)
+  int* h_done = nullptr;   // Rmax done counters, then head, tail, busy (u64)
   int* h_tail = nullptr;
   int* h_zero = nullptr;
   hipGraphExec_t graph = nullptr;
@@ -428,6 +522,7 @@ bool admit(mk_server* s) {
       q.kind = kKindImage;
       q.n_items = r.resolution * r.resolution;
       q.res = r.resolution;
+      q.tiled = (r.resolution % 8 == 0) ? 1 : 0;
       q.N = r.max_nsteps;
       q.a = rd.a; q.ci = rd.ci; q.si = rd.si; q.dist = rd.dist; q.lo = rd.lo; q.step = rd.step;
       q.rEH = rd.rEH; q.div = rd.div; q.inv_div = rd.inv_div; q.tol = rd.tol;
@@ -468,7 +563,7 @@ bool admit(mk_server* s) {
     SRV_HIP(hipMemcpyAsync(s->d_done + rid, s->h_zero, sizeof(int), hipMemcpyHostToDevice, s->stream));
     Seg& sg = s->h_segs[s->tail % s->seg_cap];
     sg.rid = rid;
-    sg.count = q.n_items;
+    sg.count = (q.n_items + kTile - 1) / kTile;
     sg.next = 0;
     sg.pad = 0;
     SRV_HIP(hipMemcpyAsync(s->d_segs + (s->tail % s->seg_cap), &sg, sizeof sg, hipMemcpyHostToDevice, s->stream));
@@ -490,7 +585,6 @@ bool admit(mk_server* s) {
 
 ServeArgs make_args(mk_server* s) {
   ServeArgs A;
-  A.req = s->d_req;
   A.done = s->d_done;
   A.segs = s->d_segs;
   A.seg_head = s->d_head;
@@ -498,6 +592,7 @@ ServeArgs make_args(mk_server* s) {
   A.seg_cap = s->seg_cap;
   A.f = s->d_f;
   A.i = s->d_i;
+  A.busy = (unsigned long long*)(s->d_head + 2);
   A.L = s->L;
   A.M = s->M;
   return A;
@@ -506,11 +601,11 @@ ServeArgs make_args(mk_server* s) {
 bool enqueue_iteration(mk_server* s) {
   const ServeArgs A = make_args(s);
   const unsigned blocks = (unsigned)((s->L + 255) / 256);
-  if (s->cfg.precision == 0) k_serve<false><<<blocks, 256, 0, s->stream>>>(A);
-  else k_serve<true><<<blocks, 256, 0, s->stream>>>(A);
+  if (s->cfg.precision == 0) k_serve<false><<<blocks, 256, 0, s->stream>>>(s->d_req, A);
+  else k_serve<true><<<blocks, 256, 0, s->stream>>>(s->d_req, A);
   SRV_HIP(hipGetLastError());
   SRV_HIP(hipMemcpyAsync(s->h_done, s->d_done, s->Rmax * sizeof(int), hipMemcpyDeviceToHost, s->stream));
-  SRV_HIP(hipMemcpyAsync(s->h_done + s->Rmax, s->d_head, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  SRV_HIP(hipMemcpyAsync(s->h_done + s->Rmax, s->d_head, 4 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
   return true;
 }
 
@@ -539,12 +634,16 @@ bool iterate(mk_server* s) {
   SRV_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
   s->st.gpu_ms += ms;
   ++s->st.iterations;
+  s->st.lane_steps_total += (int64_t)s->L * s->M;
   return true;
 }
 
 // Finished requests: copy out, free, report (caller holds mu).
 bool collect(mk_server* s) {
   s->head = s->h_done[s->Rmax];
+  unsigned long long busy;
+  std::memcpy(&busy, s->h_done + s->Rmax + 2, sizeof busy);
+  s->st.lane_steps_busy = (int64_t)busy;
   std::vector<int> fin;
   for (auto& kv : s->flight)
     if (s->h_done[kv.first] >= kv.second.n_items) fin.push_back(kv.first);
@@ -613,6 +712,7 @@ int mk_server_create(const mk_server_config* cfg, mk_server** out) {
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   s->dev = dev;
   s->L = cfg->lanes > 0 ? cfg->lanes : std::max(cus, 1) * 4 * 2 * 64;
+  s->L = (s->L + 255) / 256 * 256;  // whole blocks: every wave is full
   s->M = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : 256;
   s->Rmax = cfg->max_requests > 0 ? cfg->max_requests : 1024;
   s->arena_bytes = cfg->hbm_budget > 0 ? cfg->hbm_budget : ((size_t)4 << 30);
@@ -621,14 +721,14 @@ int mk_server_create(const mk_server_config* cfg, mk_server** out) {
   if (e == hipSuccess) e = hipMalloc(&s->d_req, s->Rmax * sizeof(ReqDev));
   if (e == hipSuccess) e = hipMalloc(&s->d_done, s->Rmax * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&s->d_segs, s->seg_cap * sizeof(Seg));
-  if (e == hipSuccess) e = hipMalloc(&s->d_head, 2 * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&s->d_head, 4 * sizeof(int));
   if (e == hipSuccess) s->d_tail = s->d_head + 1;
   if (e == hipSuccess) e = hipMalloc(&s->d_f, (size_t)kSlotF * L * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&s->d_i, (size_t)kSlotI * L * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&s->d_arena, s->arena_bytes);
   if (e == hipSuccess) e = hipHostMalloc(&s->h_req, s->Rmax * sizeof(ReqDev), hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc(&s->h_segs, s->seg_cap * sizeof(Seg), hipHostMallocDefault);
-  if (e == hipSuccess) e = hipHostMalloc(&s->h_done, (s->Rmax + 1) * sizeof(int), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&s->h_done, (s->Rmax + 4) * sizeof(int), hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc(&s->h_tail, 2 * sizeof(int), hipHostMallocDefault);
   if (e == hipSuccess) {
     s->h_zero = s->h_tail + 1;
@@ -641,7 +741,7 @@ int mk_server_create(const mk_server_config* cfg, mk_server** out) {
     // slots empty (rid = -1), counters zero: fill kernels, no memset nodes
     k_fill_int<<<grid_for(kSlotI * L, 256), 256, 0, s->stream>>>(s->d_i, kSlotI * L, -1);
     k_fill_int<<<grid_for(s->Rmax, 256), 256, 0, s->stream>>>(s->d_done, s->Rmax, 0);
-    k_fill_int<<<1, 64, 0, s->stream>>>(s->d_head, 2, 0);
+    k_fill_int<<<1, 64, 0, s->stream>>>(s->d_head, 4, 0);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
@@ -757,6 +857,8 @@ int mk_server_get_stats(mk_server* s, mk_server_stats* out) {
   out->queued = (int64_t)s->waiting.size();
   out->in_flight = (int64_t)s->flight.size();
   out->hbm_in_use = s->in_use;
+  out->lanes = s->L;
+  out->steps_per_launch = s->M;
   return server_error(s);
 }
 

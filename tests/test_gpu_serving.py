@@ -65,6 +65,8 @@ def test_served_images_equal_render_and_do_not_depend_on_the_mix(models, precisi
         st = srv.stats()
     assert st["completed"] == len(MIX) + 1 and st["in_flight"] == 0 and st["hbm_in_use"] == 0
     for k, (res, spin, inc, nu, s) in enumerate(MIX):
+        d = np.abs(alone[k] - mixed[k])
+        print(f"[serving] request {k}: alone vs mixed max abs diff {d.max():.3e} at {np.unravel_index(d.argmax(), d.shape)}")
         assert np.array_equal(alone[k], mixed[k]), k
         # same per-ray arithmetic as mk_render; the two kernels are compiled
         # separately, so FMA contraction may differ by an ulp per operation
