@@ -130,11 +130,104 @@ def serve_stream(models, precision, seconds, rate, seed=0, log=None):
     return out
 
 
+def _heavy_snapshot(seed):
+    """Turbulent torus snapshot number `seed` at its own spin, U[-0.98, 0.98]."""
+    from mahakala_amd.grmhd.synthetic import torus_snapshot
+    a = float(np.random.Generator(np.random.PCG64(1000 + seed)).uniform(-0.98, 0.98))
+    return a, torus_snapshot(a, layout="uniform", seed=seed, turbulence_sigma=0.5)
+
+
+def heavy_stream(models, seed=1, n_images=128, n_shadow=128, res=512, n_angles=4096, log=None):
+    """256 queued requests against 8 resident snapshots: 512^2 images at 8
+    frequencies log-spaced over 86-690 GHz (each image at its snapshot's spin
+    and a random inclination) and 4096-angle shadow bisections at random spin
+    and inclination, run once in mixed and once in plain FP64.  Reports
+    throughput of both and the per-image flux error of mixed against plain
+    FP64."""
+    from mahakala_amd.serving import Server
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    freqs = np.geomspace(86e9, 690e9, 8)
+    kinds = rng.permutation(np.r_[np.zeros(n_images, int), np.ones(n_shadow, int)])
+    spin = rng.uniform(-0.98, 0.98, kinds.size)
+    inc = rng.uniform(5.0, 85.0, kinds.size)
+    nu = rng.choice(freqs, kinds.size)
+    snap = rng.integers(0, len(models), kinds.size)
+    angles = np.arange(n_angles) / n_angles * 2 * np.pi
+    res_out, lat = {}, {}
+    for precision in ("mixed", "plain64"):
+        with Server(precision, use_graph=True) as srv:
+            sids = [srv.add_snapshot(m) for m in models]
+            srv.result(srv.submit_image(sids[0], 0.5, 30.0, resolution=16), timeout=120)
+            srv.poll()
+            st0 = srv.stats()
+            t0 = time.perf_counter()
+            ids = []
+            for k in range(kinds.size):
+                if kinds[k] == 0:
+                    ids.append(srv.submit_image(sids[snap[k]], models[snap[k]].bhspin, float(inc[k]), resolution=res,
+                                                observing_frequency=float(nu[k]), fluid_gamma=13. / 9))
+                else:
+                    ids.append(srv.submit_shadow(float(spin[k]), float(inc[k]), angles))
+            srv.wait(-1, timeout=900)
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+            done = dict(srv.poll(cap=kinds.size + 16))
+            res_out[precision] = [srv.result(i, 1) for i in ids]
+            lat[precision] = dict(wall=wall, busy=(st["lane_steps_busy"] - st0["lane_steps_busy"])
+                                  / max(1, st["lane_steps_total"] - st0["lane_steps_total"]),
+                                  gpu=(st["gpu_ms"] - st0["gpu_ms"]) / 1e3,
+                                  lat=np.array([done[i] for i in ids]))
+        if log:
+            print(f"[bench] heavy {precision}: {lat[precision]['wall']:.2f} s", file=log, flush=True)
+    img = np.flatnonzero(kinds == 0)
+    sh = np.flatnonzero(kinds == 1)
+    sums = {p: np.array([float(res_out[p][k].sum()) for k in img]) for p in res_out}
+    with np.errstate(all="ignore"):
+        flux_err = np.abs(sums["mixed"] - sums["plain64"]) / np.abs(sums["plain64"])
+    # The reference's explicit-Euler transfer, I += -dt L (j - alpha I), is
+    # unstable where alpha |dt| L > 2 (optically thick steps, here the
+    # low-frequency images of the densest turbulent snapshots): intensities
+    # then turn negative and grow without bound in either precision, and the
+    # two precisions diverge.  Such images are counted apart.
+    stable = np.array([np.all(res_out["plain64"][k] >= 0) and np.all(np.isfinite(res_out["plain64"][k]))
+                       for k in img])
+    bad = [dict(spin=round(float(models[snap[k]].bhspin), 4), inc=round(float(inc[k]), 2), nu=float(nu[k]), snapshot=int(snap[k]),
+                flux_mixed=float(sums["mixed"][n]), flux_plain=float(sums["plain64"][n]),
+                nonfinite_mixed=int(np.count_nonzero(~np.isfinite(res_out["mixed"][k]))),
+                nonfinite_plain=int(np.count_nonzero(~np.isfinite(res_out["plain64"][k]))),
+                max_mixed=float(np.nanmax(res_out["mixed"][k])), max_plain=float(np.nanmax(res_out["plain64"][k])))
+           for n, k in enumerate(img) if stable[n] and not flux_err[n] <= 1e-3]
+    same_radii = all(np.array_equal(res_out["mixed"][k], res_out["plain64"][k]) for k in sh)
+    iters = 14  # bracket [0.5, 10] halved to 1e-3
+    out = {"workload": f"{kinds.size} queued requests against {len(models)} resident 512-block torus snapshots "
+                       f"with log-normal turbulence (sigma 0.5, seeds 0-{len(models) - 1}, spins "
+                       f"{[round(m.bhspin, 3) for m in models]}): {n_images} images of {res}^2 at 8 frequencies "
+                       f"86-690 GHz at their snapshot's spin, and {n_shadow} shadow bisections of {n_angles} angles "
+                       "at spin U[-0.98,0.98]; inclination U[5,85] deg",
+           "image_rays": int(img.size * res * res), "bisection_ray_integrations": int(sh.size * n_angles * iters)}
+    for precision in ("mixed", "plain64"):
+        L = lat[precision]
+        out[precision] = {"wall_s": round(L["wall"], 3), "gpu_busy_s": round(L["gpu"], 3),
+                          "lane_utilisation": round(L["busy"], 4),
+                          "requests_per_s": round(kinds.size / L["wall"], 2),
+                          "latency_ms": {"p50": round(float(np.percentile(L["lat"], 50)), 2),
+                                         "p99": round(float(np.percentile(L["lat"], 99)), 2)}}
+    out["speedup_mixed_over_plain"] = round(lat["plain64"]["wall"] / lat["mixed"]["wall"], 3)
+    out["image_flux_rel_err_mixed_vs_plain"] = {
+        "stable_images": int(stable.sum()), "max": float(np.max(flux_err[stable])) if stable.any() else None,
+        "median": float(np.median(flux_err[stable])) if stable.any() else None, "stable_images_above_1e-3": bad,
+        "unstable_images": int((~stable).sum()),
+        "unstable_median": float(np.median(flux_err[~stable])) if (~stable).any() else None}
+    out["shadow_radii_identical"] = bool(same_radii)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--res", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--only", choices=["mixed", "plain64", "mixed-zone"], default=None,
@@ -142,6 +235,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="length of the Poisson serving stream")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean requests/s of the serving stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--no-heavy", action="store_true", help="skip the heavy queued stream (8 snapshots)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the device copy kernel over MIB MiB 3 times (traffic counter calibration)")
     args = ap.parse_args()
@@ -273,6 +367,17 @@ def main():
         out["serving"] = sv
         out["latency_ms"] = dict(sv["latency_ms"], what="submit-to-result latency of the serving stream")
         del model1, ds1
+
+    if not args.no_heavy:
+        from concurrent.futures import ProcessPoolExecutor
+        t1 = time.perf_counter()
+        with ProcessPoolExecutor(max_workers=8) as ex:
+            dss = list(ex.map(_heavy_snapshot, range(8)))
+        models8 = [AthenakFluidModel.from_datasets(d, a8, fluid_gamma=13. / 9) for a8, d in dss]  # uploads
+        del dss
+        print(f"[bench] 8 heavy snapshots ready in {time.perf_counter() - t1:.1f}s", file=sys.stderr, flush=True)
+        out["serving_heavy"] = heavy_stream(models8, log=sys.stderr)
+        del models8
 
     if not args.no_cpu:
         from oracle import c_oracle

@@ -91,3 +91,40 @@ def test_make_image_fused_equals_render(torus):
     img = images.make_image(torus, camera_inclination=30, resolution=24)
     p = images.render_params(0.5, camera_inclination=30, resolution=24)
     np.testing.assert_array_equal(img, images.render(torus.device_snapshot, p))
+
+
+@pytest.fixture(scope="module")
+def octree():
+    """3-level 512-leaf tree over +-50 M with log-normal turbulence (a = 0.94)."""
+    ds = syn.config3_snapshot(0.94, seed=0)
+    return AthenakFluidModel.from_datasets(ds, 0.94, fluid_gamma=13. / 9)
+
+
+def test_octree_turbulence_plain_fp64_vs_c_restatement(octree):
+    """a = 0.94, i = 17 deg on the refined tree: block lookup across three
+    levels, halos from coarser and finer neighbours, and the turbulent field."""
+    res = 40
+    p = images.render_params(0.94, camera_inclination=17, fov=20, resolution=res)
+    img, steps = images.render(octree.device_snapshot, p, return_steps=True)
+    cs = helpers.c_snapshot_from_model(octree)
+    ref, ref_steps = cs.render(0.94, 17, 1000.0, 20.0, res, 10000, 13. / 9, 40.0, 1e26, 6.2e9 * MSUN, 230e9)
+    ref = ref.reshape(res, res)
+    assert ref.sum() > 0 and np.all(ref >= 0)
+    rel, mask = _pixel_close(img, ref)
+    assert mask.sum() > res and np.mean(rel < 1e-4) >= 0.999 and np.median(rel) < 1e-8
+    assert abs(img.sum() - ref.sum()) / ref.sum() < 1e-5
+    assert np.mean(steps.ravel() == ref_steps) >= 0.999
+
+
+@pytest.mark.parametrize("precision", ["mixed-zone", "mixed"])
+def test_octree_1024_mixed_vs_plain_flux(octree, precision):
+    """1024^2 rays, a = 0.94, i = 17 deg, 230 GHz: total flux of each mixed
+    preset within 1e-3 of plain FP64."""
+    res = 1024
+    plain = images.render(octree.device_snapshot, images.render_params(0.94, camera_inclination=17, resolution=res))
+    mixed = images.render(octree.device_snapshot, images.render_params(0.94, camera_inclination=17, resolution=res,
+                                                                        precision=precision))
+    assert plain.sum() > 0 and np.all(plain >= 0)
+    err = abs(mixed.sum() - plain.sum()) / plain.sum()
+    print(f"[render] 1024^2 octree {precision}: flux rel err {err:.3e}")
+    assert err < 1e-3
