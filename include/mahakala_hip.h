@@ -190,9 +190,10 @@ int mk_render(const mk_snapshot* snap, const mk_render_params* params, double* d
  * requests of different size, spin, inclination and frequency, in shared
  * launches of packed rays.  A worker thread admits queued requests while
  * their output buffers fit the HBM budget, and launches an advance kernel in
- * which every lane runs up to steps_per_launch RK4 steps and, when its ray or
- * bisection finishes, takes the next work item of any admitted request.
- * Results do not depend on what else is in flight. */
+ * which every wave runs up to steps_per_launch RK4 steps on a tile of 64
+ * rays or bisection angles of one request and, when the tile is done, takes
+ * the next tile of any admitted request.  Results do not depend on what else
+ * is in flight. */
 typedef struct mk_server mk_server;
 
 typedef struct mk_server_config {
