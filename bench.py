@@ -236,6 +236,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean requests/s of the serving stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-heavy", action="store_true", help="skip the heavy queued stream (8 snapshots)")
+    ap.add_argument("--no-target", action="store_true", help="skip the a=0.9 precision-target image")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the device copy kernel over MIB MiB 3 times (traffic counter calibration)")
     args = ap.parse_args()
@@ -358,6 +359,35 @@ def main():
                                  "speedup_over_plain": round(pw / zw, 3),
                                  "accuracy_vs_plain": {k: (round(v, 9) if isinstance(v, float) else v)
                                                        for k, v in accuracy(I_mix["mixed-zone"]).items()}}
+
+    if not args.no_target:
+        # the precision target's case: a = 0.9, 256^2, i = 60 deg (make_image's
+        # default camera), torus generated at a = 0.9 on the 512-block layout
+        ds9 = config2_snapshot(0.9, seed=0)
+        m9 = AthenakFluidModel.from_datasets(ds9, 0.9, fluid_gamma=13. / 9)
+        del ds9
+        img9 = L.DeviceBuffer(8 * n)
+        tgt = {}
+        for p in ("plain64", "mixed", "mixed-zone"):
+            rp9 = images.render_params(0.9, camera_inclination=60, fov=fov, resolution=res, observing_frequency=nu,
+                                       precision=p)
+
+            def fn9(rp9=rp9):
+                L.check(L.lib().mk_render(m9.device_snapshot.handle, rp9, img9.ptr, None, None))
+            fn9()
+            L.sync()
+            I9 = img9.to_array((res, res))
+            ms9, w9 = time_launches(L, fn9, args.steps, args.warmup)
+            tgt[p] = dict(ms=float(np.mean(ms9)), I=I9)
+        f9 = float(tgt["plain64"]["I"].sum())
+        out["target_a0.9"] = {
+            "what": "256^2, a=0.9, i=60 deg, 230 GHz, torus at a=0.9 on 512 blocks of 32^3 over +-40 M",
+            "plain64_ms": round(tgt["plain64"]["ms"], 4),
+            **{p: {"ms": round(tgt[p]["ms"], 4), "speedup_over_plain": round(tgt["plain64"]["ms"] / tgt[p]["ms"], 3),
+                   "flux_rel_err": abs(float(tgt[p]["I"].sum()) - f9) / f9} for p in ("mixed", "mixed-zone")}}
+        print(f"[bench] a=0.9 target: plain {tgt['plain64']['ms']:.3f} ms, mixed {tgt['mixed']['ms']:.3f} ms",
+              file=sys.stderr, flush=True)
+        del m9, img9
 
     if not args.no_serve:
         ds1 = config2_snapshot(a, seed=1)

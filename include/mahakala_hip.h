@@ -239,9 +239,11 @@ int mk_server_add_snapshot(mk_server* srv, const mk_snapshot* snap, int32_t* h_i
 int mk_server_submit_image(mk_server* srv, const mk_image_request* req, double* h_out, int64_t* h_id);
 /* queue a shadow bisection; h_out (n_angles doubles) written on completion */
 int mk_server_submit_shadow(mk_server* srv, const mk_shadow_request* req, double* h_out, int64_t* h_id);
-/* completed requests since the last poll: ids and submit-to-done latency */
+/* completed requests since the last poll: ids and submit-to-done latency
+ * (the newest 2^20 completions are kept between polls) */
 int mk_server_poll(mk_server* srv, int64_t* h_ids, double* h_latency_ms, int32_t cap, int32_t* h_n);
-/* block until request id (or all, id < 0) completes; MK_ERR_TIMEOUT */
+/* block until request id (or all, id < 0) completes; MK_ERR_TIMEOUT after
+ * timeout_s seconds, MK_ERR_ARG for an id never issued */
 int mk_server_wait(mk_server* srv, int64_t id, double timeout_s);
 int mk_server_get_stats(mk_server* srv, mk_server_stats* h_stats);
 int mk_server_destroy(mk_server* srv);

@@ -25,7 +25,6 @@
 #include <deque>
 #include <map>
 #include <mutex>
-#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -429,7 +428,6 @@ struct mk_server {
   std::map<int, Flight> flight;
   std::vector<int> free_rids;
   std::vector<std::pair<int64_t, double>> unpolled;
-  std::set<int64_t> done_ids;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::thread worker;
@@ -444,6 +442,8 @@ struct mk_server {
 namespace {
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+constexpr size_t kMaxUnpolled = size_t(1) << 20;  // completions kept for mk_server_poll
 
 bool arena_alloc(mk_server* s, size_t bytes, size_t* off) {
   for (auto it = s->free_blocks.begin(); it != s->free_blocks.end(); ++it) {
@@ -657,8 +657,9 @@ bool collect(mk_server* s) {
     arena_free(s, f.off, f.bytes);
     s->flight.erase(rid);
     s->free_rids.push_back(rid);
+    if (s->unpolled.size() >= kMaxUnpolled)  // keep the newest completions
+      s->unpolled.erase(s->unpolled.begin(), s->unpolled.begin() + kMaxUnpolled / 2);
     s->unpolled.emplace_back(id, ms);
-    s->done_ids.insert(id);
     ++s->st.completed;
     s->st.items_done += n_items;
   }
@@ -838,10 +839,20 @@ int mk_server_poll(mk_server* s, int64_t* ids, double* lat, int32_t cap, int32_t
 int mk_server_wait(mk_server* s, int64_t id, double timeout_s) {
   MK_REQUIRE(s, "mk_server_wait: null server");
   std::unique_lock<std::mutex> lk(s->mu);
+  if (id >= s->next_id || id == 0) {
+    set_error("mk_server_wait: unknown request id %lld", (long long)id);
+    return MK_ERR_ARG;
+  }
+  // a known id is done once it is neither queued nor in flight, so no
+  // per-request record outlives the request
   auto ready = [&] {
     if (s->status != MK_OK) return true;
-    if (id >= 0) return s->done_ids.count(id) > 0;
-    return s->waiting.empty() && s->flight.empty();
+    if (id < 0) return s->waiting.empty() && s->flight.empty();
+    for (const auto& p : s->waiting)
+      if (p.id == id) return false;
+    for (const auto& kv : s->flight)
+      if (kv.second.id == id) return false;
+    return true;
   };
   if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
     set_error("mk_server_wait: timed out after %.3f s", timeout_s);

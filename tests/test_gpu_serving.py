@@ -128,6 +128,9 @@ def test_budget_back_pressure_and_errors(models):
         done = srv.poll()
         assert sorted(i for i, _ in done) == sorted(rids)
         assert all(ms > 0 for _, ms in done)
+        srv.wait(rids[0], timeout=0.0)  # finished and polled: returns at once
+        with pytest.raises(_lib.MkError, match="unknown request"):
+            srv.wait(10 ** 6, timeout=0.0)
     with Server("plain64") as srv:
         sid = srv.add_snapshot(models[0])
         rid = srv.submit_image(sid, **_img_kw(256, 0.5, 30.0, 230e9))
