@@ -334,44 +334,76 @@ struct SnapView {
   double g0[3], ginv[3], dom_lo[3], dom_hi[3];
 };
 
-__device__ __forceinline__ bool in_block(const double* mb, double x, double y, double z) {
-  return (mb[0] < x && x <= mb[3]) && (mb[1] < y && y <= mb[4]) && (mb[2] < z && z <= mb[5]);
+// Left-open, right-closed box test on a block record (lo[3], hi[3], ...),
+// evaluated without short-circuit so that the six bounds load together.
+__device__ __forceinline__ bool in_box(const double* r, double x, double y, double z) {
+  const double l0 = r[0], l1 = r[1], l2 = r[2], h0 = r[3], h1 = r[4], h2 = r[5];
+  return (l0 < x) & (x <= h0) & (l1 < y) & (y <= h1) & (l2 < z) & (z <= h2);
 }
 
-// Leaf id with the reference's rule (left-open, right-closed, last match
-// wins; athenak.py:657-670), via the level-resolution lookup grid.  `meta`
-// and `grid` are the snapshot's tables, in global memory or copied to LDS.
-__device__ __forceinline__ int find_block(const SnapView& sv, const double* meta, const int* grid, double x,
-                                          double y, double z) {
-  if (!(sv.dom_lo[0] < x && x <= sv.dom_hi[0] && sv.dom_lo[1] < y && y <= sv.dom_hi[1] && sv.dom_lo[2] < z &&
-        z <= sv.dom_hi[2]))
-    return -1;
-  int c[3];
-  const double p[3] = {x, y, z};
-  const int gn[3] = {sv.gx, sv.gy, sv.gz};
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    int ci = (int)__builtin_ceil((p[d] - sv.g0[d]) * sv.ginv[d]) - 1;
-    ci = ci < 0 ? 0 : (ci >= gn[d] ? gn[d] - 1 : ci);
-    c[d] = ci;
-  }
-  {
-    const int b = grid[((long)c[2] * sv.gy + c[1]) * sv.gx + c[0]];
-    if (b >= 0 && in_block(meta + kMeta * (long)b, x, y, z)) return b;
-  }
-  // within rounding of a face: the neighbouring cells
+__device__ __forceinline__ bool in_block(const double* mb, double x, double y, double z) { return in_box(mb, x, y, z); }
+
+// Slow part of the leaf lookup: the 26 neighbouring lookup cells (a point
+// within rounding of a face), then a scan of every block (holes in the
+// tiling), as the reference's brute-force rule.
+__device__ __forceinline__ int find_block_slow(const SnapView& sv, const double* meta, const int* grid, const int c[3],
+                                               double x, double y, double z) {
+#pragma unroll 1
   for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll 1
     for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll 1
       for (int dx = -1; dx <= 1; ++dx) {
         const int cx = c[0] + dx, cy = c[1] + dy, cz = c[2] + dz;
         if (cx < 0 || cy < 0 || cz < 0 || cx >= sv.gx || cy >= sv.gy || cz >= sv.gz) continue;
         const int b = grid[((long)cz * sv.gy + cy) * sv.gx + cx];
         if (b >= 0 && in_block(meta + kMeta * (long)b, x, y, z)) return b;
       }
-  int found = -1;  // holes in the tiling: scan, as the reference does
+  int found = -1;
+#pragma unroll 1
   for (int b = 0; b < sv.nmb; ++b)
     if (in_block(meta + kMeta * (long)b, x, y, z)) found = b;
   return found;
+}
+
+// Leaf id with the reference's rule (left-open, right-closed, last match
+// wins; athenak.py:657-670), via the level-resolution lookup grid, and the
+// block's record (lo, hi, q0, 1/dx) in rec.  `meta` and `grid` are the
+// snapshot's tables, in global memory or copied to LDS.  The common case is
+// one lookup-grid read followed by one 96-byte record read.
+__device__ __forceinline__ int find_block_rec(const SnapView& sv, const double* meta, const int* grid, double x,
+                                              double y, double z, double rec[kMeta]) {
+  const bool inside = (sv.dom_lo[0] < x) & (x <= sv.dom_hi[0]) & (sv.dom_lo[1] < y) & (y <= sv.dom_hi[1]) &
+                      (sv.dom_lo[2] < z) & (z <= sv.dom_hi[2]);
+  int c[3];
+  const double p[3] = {x, y, z};
+  const int gn[3] = {sv.gx, sv.gy, sv.gz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double t = inside ? (p[d] - sv.g0[d]) * sv.ginv[d] : 0.0;
+    int ci = (int)__builtin_ceil(t) - 1;
+    ci = ci < 0 ? 0 : (ci >= gn[d] ? gn[d] - 1 : ci);
+    c[d] = ci;
+  }
+  int b = grid[((long)c[2] * sv.gy + c[1]) * sv.gx + c[0]];
+  const double* m = meta + kMeta * (long)(b < 0 ? 0 : b);
+#pragma unroll
+  for (int q = 0; q < kMeta; ++q) rec[q] = m[q];
+  const bool hit = inside & (b >= 0) & in_box(rec, x, y, z);
+  if (!hit) {
+    b = inside ? find_block_slow(sv, meta, grid, c, x, y, z) : -1;
+    if (b >= 0) {
+#pragma unroll
+      for (int q = 0; q < kMeta; ++q) rec[q] = meta[kMeta * (long)b + q];
+    }
+  }
+  return b;
+}
+
+__device__ __forceinline__ int find_block(const SnapView& sv, const double* meta, const int* grid, double x,
+                                          double y, double z) {
+  double rec[kMeta];
+  return find_block_rec(sv, meta, grid, x, y, z, rec);
 }
 
 template <typename T>
@@ -391,11 +423,11 @@ struct Stencil {
 __device__ __forceinline__ Stencil locate(const SnapView& sv, const double* meta, const int* grid, double x,
                                           double y, double z) {
   Stencil st;
-  st.mb = find_block(sv, meta, grid, x, y, z);
+  double m[kMeta];
+  st.mb = find_block_rec(sv, meta, grid, x, y, z, m);
   st.base = 0;
   st.fr[0] = st.fr[1] = st.fr[2] = 0.0;
   if (st.mb < 0) return st;
-  const double* m = meta + kMeta * (long)st.mb;
   const double p[3] = {x, y, z};
   int id[3];
 #pragma unroll
