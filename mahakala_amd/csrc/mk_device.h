@@ -465,14 +465,10 @@ __device__ __forceinline__ void fetch(const SnapView& sv, const Stencil& st, Cor
       }
 }
 
-// Interpolate x1, then x2, then x3 (athenak.py:746-752); zero outside.
+// Interpolate x1, then x2, then x3 (athenak.py:746-752), whatever the
+// stencil (callers discard the result of a stencil outside the domain).
 template <typename T>
-__device__ __forceinline__ void interp(const Stencil& st, const Corners& c, T P[8]) {
-  if (st.mb < 0) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) P[k] = T(0);
-    return;
-  }
+__device__ __forceinline__ void interp_raw(const Stencil& st, const Corners& c, T P[8]) {
   const T f0 = (T)st.fr[0], f1 = (T)st.fr[1], f2 = (T)st.fr[2];
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
@@ -492,6 +488,17 @@ __device__ __forceinline__ void interp(const Stencil& st, const Corners& c, T P[
     const T db = lerp_ref(dba, dbb, f1);
     P[v] = lerp_ref(da, db, f2);
   }
+}
+
+// Interpolated primitives; zero outside the domain.
+template <typename T>
+__device__ __forceinline__ void interp(const Stencil& st, const Corners& c, T P[8]) {
+  if (st.mb < 0) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) P[k] = T(0);
+    return;
+  }
+  interp_raw<T>(st, c, P);
 }
 
 template <typename T>

@@ -57,21 +57,46 @@ __device__ __forceinline__ void trace_ray(const SnapView& sv, const double* meta
     I += trans * (c * (double)j);
     trans *= (1.0 - c * (double)al);
   };
+  // The same sum without branches: lanes with nothing to add evaluate the
+  // emission too and keep I and the transmission by selection.
+  auto consume_flat = [&]() {
+    T P[8], j, al;
+    interp_raw<T>(pst, pc, P);
+    emission_from_prims<T>(rp.ip, P, ps, rp.a, j, al);
+    const bool use = pend && pst.mb >= 0 && P[0] != T(0);
+    const double c = -pdt * rp.ip.L_unit;
+    const double In = I + trans * (c * (double)j);
+    const double tn = trans * (1.0 - c * (double)al);
+    I = use ? In : I;
+    trans = use ? tn : trans;
+  };
   const bool zone = MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) pc.q[q][0] = pc.q[q][1] = make_float4(0.f, 0.f, 0.f, 0.f);
+  pst.mb = -1;
+  pst.base = 0;
+  pst.fr[0] = pst.fr[1] = pst.fr[2] = 0.0;
   for (int k = 0; k < rp.N; ++k) {
     if (dt == 0.0) break;
     double nw[8];
-    if constexpr (!MIXED) {
-      rk4_step<double>(s, dt, rp.a, nw);
-    } else {
-      const bool fp64 = zone && ((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
-      if (fp64) rk4_step<double>(s, dt, rp.a, nw);
+    if (!zone && __any(pend && pst.mb >= 0)) {
+      // a sample of the wave is inside the domain: the next RK4 step and
+      // that sample's emission are independent, and one straight block lets
+      // a wave that is alone on its SIMD interleave the two
+      if constexpr (!MIXED) rk4_step<double>(s, dt, rp.a, nw);
       else rk4_step_mixed(s, dt, (float)rp.a, nw);
+      consume_flat();
+    } else {
+      if constexpr (!MIXED) {
+        rk4_step<double>(s, dt, rp.a, nw);
+      } else {
+        const bool fp64 = zone && ((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
+        if (fp64) rk4_step<double>(s, dt, rp.a, nw);
+        else rk4_step_mixed(s, dt, (float)rp.a, nw);
+      }
+      if (pend) consume();
     }
-    if (pend) {
-      consume();
-      pend = false;
-    }
+    pend = false;
     double rn, dtn;
     if constexpr (!MIXED) {
       rn = radius_cal(nw[1], nw[2], nw[3], rp.a);
