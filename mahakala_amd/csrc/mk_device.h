@@ -56,6 +56,11 @@ template <> struct Math<double> {
     return x == 0.0 ? x : g;
   }
   static __device__ __forceinline__ double sqrt_ieee(double x) { return __builtin_sqrt(x); }
+  // sqrt(x) and 1/sqrt(x)
+  static __device__ __forceinline__ void sqrt_rcp(double x, double& s, double& is) {
+    s = sqrt(x);
+    is = rcp(s);
+  }
   static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
   static __device__ __forceinline__ double pow(double x, double y) { return ::pow(x, y); }
   static __device__ __forceinline__ double cbrt(double x) { return ::cbrt(x); }
@@ -69,6 +74,11 @@ template <> struct Math<float> {
   static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float sqrt_ieee(float x) { return __builtin_amdgcn_sqrtf(x); }
+  // sqrt(x) and 1/sqrt(x) from one hardware reciprocal square root
+  static __device__ __forceinline__ void sqrt_rcp(float x, float& s, float& is) {
+    is = __builtin_amdgcn_rsqf(x);
+    s = x * is;
+  }
   static __device__ __forceinline__ float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
   static __device__ __forceinline__ float pow(float x, float y) {
     // x >= 0 here; log2(0) = -inf gives 0, log2(inf) = inf gives inf
@@ -125,11 +135,10 @@ __device__ __forceinline__ void ks_accel(T x, T y, T z, T v0, T v1, T v2, T v3, 
   using M = Math<T>;
   const T aa = a * a, zz = z * z;
   const T kk = T(0.5) * (x * x + y * y + zz - aa);
-  const T Q = M::sqrt(kk * kk + aa * zz);
-  const T iQ = M::rcp(Q);
+  T Q, iQ, r, ir;
+  M::sqrt_rcp(kk * kk + aa * zz, Q, iQ);
   const T rr = Q + kk;
-  const T r = M::sqrt(rr);
-  const T ir = M::rcp(r);
+  M::sqrt_rcp(rr, r, ir);
   const T iden = M::rcp(rr * rr + aa * zz);
   const T f = (T(2) * rr * r) * iden;
   const T s = M::rcp(rr + aa);
@@ -230,13 +239,14 @@ __device__ __forceinline__ double radius_cal(double x, double y, double z, doubl
   return __builtin_sqrt((R2 - aa + __builtin_sqrt((R2 - aa) * (R2 - aa) + 4.0 * aa * (z * z))) / 2.0);
 }
 
-// radius_cal with R^2 = x^2 + y^2 + z^2 taken directly and the refined
-// hardware square root (within an ulp or two of radius_cal)
-__device__ __forceinline__ double radius_fast(double x, double y, double z, double a) {
-  using M = Math<double>;
-  const double R2 = x * x + y * y + z * z, aa = a * a;
-  const double w = R2 - aa;
-  return M::sqrt(0.5 * (w + M::sqrt(w * w + 4.0 * aa * (z * z))));
+// radius_cal in FP32 from the FP64 position (the mixed step rule: FP32
+// resolves |r - r_EH| to ~1e-7 M against the 1e-4 tolerance, and r to
+// ~1e-4 M against the 1500 M escape bound)
+__device__ __forceinline__ double radius_f32(double x, double y, double z, double a) {
+  const float xf = (float)x, yf = (float)y, zf = (float)z, af = (float)a;
+  const float R2 = xf * xf + yf * yf + zf * zf, aa = af * af;
+  const float w = R2 - aa;
+  return (double)__builtin_amdgcn_sqrtf(0.5f * (w + __builtin_amdgcn_sqrtf(w * w + 4.0f * aa * (zf * zf))));
 }
 
 // step rule with the division by div replaced by a product with 1/div

@@ -102,7 +102,7 @@ __device__ __forceinline__ void trace_ray(const SnapView& sv, const double* meta
       rn = radius_cal(nw[1], nw[2], nw[3], rp.a);
       dtn = step_rule(rn, rp.rEH, rp.div, rp.tol);
     } else {
-      rn = radius_fast(nw[1], nw[2], nw[3], rp.a);
+      rn = radius_f32(nw[1], nw[2], nw[3], rp.a);
       dtn = step_rule_fast(rn, rp.rEH, rp.inv_div, rp.div, rp.tol);
     }
     if (dtn == 0.0) break;

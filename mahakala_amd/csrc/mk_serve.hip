@@ -203,7 +203,7 @@ __device__ __forceinline__ bool image_pass(const ReqDev& q, bool go, Lane& L, Pe
       rn = radius_cal(nw[1], nw[2], nw[3], q.a);
       dtn = step_rule(rn, q.rEH, q.div, q.tol);
     } else {
-      rn = radius_fast(nw[1], nw[2], nw[3], q.a);
+      rn = radius_f32(nw[1], nw[2], nw[3], q.a);
       dtn = step_rule_fast(rn, q.rEH, q.inv_div, q.div, q.tol);
     }
     if (dtn == 0.0) {
