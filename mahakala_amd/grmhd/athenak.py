@@ -11,6 +11,7 @@ from an .athdf file), which is how the synthetic snapshots are loaded.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import time
 
 import numpy as np
@@ -34,12 +35,21 @@ def vec_imetric(X, bhspin):
 
 
 def read_athdf(filename):
-    """Datasets of an AthenaK .athdf file as a dict (athenak.py:79-103)."""
+    """Datasets of an AthenaK .athdf file as a dict (athenak.py:79-103).
+
+    Uses h5py when it is importable, else the native reader
+    (libmk_athdf.so, the HDF5 C library); the two give the same arrays."""
     try:
         import h5py
-    except ImportError as exc:  # the image's main interpreter has no h5py
-        raise ImportError("reading .athdf files needs h5py; build the model with "
-                          "AthenakFluidModel.from_datasets instead") from exc
+    except ImportError:  # the image's main interpreter has no h5py
+        from mahakala_amd import _athdf
+        try:
+            return _athdf.read_athdf_native(filename)
+        except _athdf.AthdfError as exc:
+            if not os.path.exists(_athdf.LIB_PATH):
+                raise ImportError("reading .athdf files needs h5py or libmk_athdf.so; build the model "
+                                  "with AthenakFluidModel.from_datasets instead") from exc
+            raise
     with h5py.File(filename, "r") as f:
         ds = {k: np.array(f[k]) for k in ("x1v", "x2v", "x3v", "x1f", "x2f", "x3f", "uov", "B",
                                           "LogicalLocations", "Levels")}

@@ -1,5 +1,6 @@
 """CPU checks of the C ABI and host logic (no GPU needed)."""
 import ctypes as C
+import os
 import re
 import subprocess
 
@@ -95,3 +96,17 @@ def test_server_structs_match_header():
         for m in re.finditer(r"^\s*(?:const\s+)?(?:double|int32_t|int64_t|size_t)\*?\s+([\w,\s*]+);", block, re.M):
             fields += [f.strip().lstrip("*") for f in m.group(1).split(",")]
         assert fields == [n for n, _ in getattr(_lib, name)._fields_], name
+
+
+def test_athdf_library_exports_every_declared_symbol():
+    from mahakala_amd import _athdf
+    if not os.path.exists(_athdf.LIB_PATH):
+        pytest.skip("libmk_athdf.so not built (no HDF5 headers)")
+    text = open(_athdf.HEADER_PATH).read()
+    declared = sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(mk_athdf_\w+)\s*\(", text, re.M)))
+    assert len(declared) == 6
+    out = subprocess.run(["nm", "-D", "--defined-only", _athdf.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (mk_athdf_\w+)", out))
+    assert not [s for s in declared if s not in exported]
+    _athdf.lib()
