@@ -257,11 +257,11 @@ def main():
         print(json.dumps({"copy_calibration_bytes": nb, "launches": 3}))
         return
     from mahakala_amd.grmhd.athenak import AthenakFluidModel
-    from mahakala_amd.grmhd.synthetic import config2_snapshot
+    from mahakala_amd.grmhd.synthetic import uniform_torus_snapshot
 
     a, inc, nu, fov, res = 0.5, 30.0, 230e9, 20.0, args.res
     t0 = time.perf_counter()
-    ds = config2_snapshot(a, seed=0)
+    ds = uniform_torus_snapshot(a, seed=0)
     model = AthenakFluidModel.from_datasets(ds, a, fluid_gamma=13. / 9)
     snap = model.device_snapshot
     print(f"[bench] snapshot ready in {time.perf_counter() - t0:.1f}s ({snap.nbytes / 2**20:.0f} MiB on device)",
@@ -363,7 +363,7 @@ def main():
     if not args.no_target:
         # the precision target's case: a = 0.9, 256^2, i = 60 deg (make_image's
         # default camera), torus generated at a = 0.9 on the 512-block layout
-        ds9 = config2_snapshot(0.9, seed=0)
+        ds9 = uniform_torus_snapshot(0.9, seed=0)
         m9 = AthenakFluidModel.from_datasets(ds9, 0.9, fluid_gamma=13. / 9)
         del ds9
         img9 = L.DeviceBuffer(8 * n)
@@ -390,7 +390,7 @@ def main():
         del m9, img9
 
     if not args.no_serve:
-        ds1 = config2_snapshot(a, seed=1)
+        ds1 = uniform_torus_snapshot(a, seed=1)
         model1 = AthenakFluidModel.from_datasets(ds1, a, fluid_gamma=13. / 9)
         sv = serve_stream([model, model1], head if head != "mixed-zone" else "mixed", args.serve_seconds,
                           args.serve_rate, log=sys.stderr)

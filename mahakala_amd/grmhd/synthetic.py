@@ -208,11 +208,11 @@ def torus_snapshot(bhspin, layout="uniform", seed=0, turbulence_sigma=0.0, rho0=
     return ds
 
 
-def config2_snapshot(bhspin=0.5, seed=0):
+def uniform_torus_snapshot(bhspin=0.5, seed=0):
     """256^3 uniform torus as 512 blocks of 32^3 over +-40 M."""
     return torus_snapshot(bhspin, layout="uniform", seed=seed)
 
 
-def config3_snapshot(bhspin=0.94, seed=0):
+def octree_turbulent_snapshot(bhspin=0.94, seed=0):
     """3-level 512-leaf tree over +-50 M with log-normal turbulence (sigma 0.5)."""
     return torus_snapshot(bhspin, layout="octree", seed=seed, turbulence_sigma=0.5)

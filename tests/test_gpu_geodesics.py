@@ -67,7 +67,7 @@ def test_integrator_history_vs_oracle():
     np.testing.assert_array_equal(K, f["K"])
 
 
-def test_integrate_final_config1():
+def test_integrate_final_64x64_kerr_camera():
     """64x64, a=0.9, i=60, camera at 1000 M; tol = 0.01 r_EH (the '1.01 r_h' stop)."""
     a = 0.9
     rEH = 1 + np.sqrt(1 - a * a)
@@ -103,7 +103,7 @@ def test_shadow_reference_cases():
         np.testing.assert_array_equal(radii, c_oracle.find_shadow_bisection_angles(a, inc, ang))
 
 
-def test_shadow_config1_and_select():
+def test_shadow_64_angles_and_select():
     ang, radii = ma.find_shadow_bisection(0.9, 60, 64)
     assert ang.shape == (65,) and radii[0] == radii[-1]
     ref = c_oracle.find_shadow_bisection_angles(0.9, 60, ang[:-1])

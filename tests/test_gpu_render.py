@@ -27,7 +27,7 @@ def _pixel_close(img, ref, rtol=1e-4):
 
 @pytest.fixture(scope="module")
 def torus():
-    ds = syn.config2_snapshot(0.5, seed=0)
+    ds = syn.uniform_torus_snapshot(0.5, seed=0)
     return AthenakFluidModel.from_datasets(ds, 0.5, fluid_gamma=13. / 9)
 
 
@@ -65,7 +65,7 @@ class _Unfused:
         return self._m.get_fluid_scalars_from_geodesics(S)
 
 
-def test_config2_plain_fp64_vs_c_restatement(torus):
+def test_uniform_torus_plain_fp64_vs_c_restatement(torus):
     res = 48
     p = images.render_params(0.5, camera_inclination=30, fov=20, resolution=res)
     img, steps = images.render(torus.device_snapshot, p, return_steps=True)
@@ -79,7 +79,7 @@ def test_config2_plain_fp64_vs_c_restatement(torus):
     assert np.mean(steps.ravel() == ref_steps) >= 0.999
 
 
-def test_config2_mixed_vs_plain(torus):
+def test_uniform_torus_mixed_vs_plain(torus):
     res = 64
     plain = images.render(torus.device_snapshot, images.render_params(0.5, camera_inclination=30, resolution=res))
     mixed = images.render(torus.device_snapshot,
@@ -96,7 +96,7 @@ def test_make_image_fused_equals_render(torus):
 @pytest.fixture(scope="module")
 def octree():
     """3-level 512-leaf tree over +-50 M with log-normal turbulence (a = 0.94)."""
-    ds = syn.config3_snapshot(0.94, seed=0)
+    ds = syn.octree_turbulent_snapshot(0.94, seed=0)
     return AthenakFluidModel.from_datasets(ds, 0.94, fluid_gamma=13. / 9)
 
 
@@ -128,3 +128,35 @@ def test_octree_1024_mixed_vs_plain_flux(octree, precision):
     err = abs(mixed.sum() - plain.sum()) / plain.sum()
     print(f"[render] 1024^2 octree {precision}: flux rel err {err:.3e}")
     assert err < 1e-3
+
+
+@pytest.mark.parametrize("res,nsteps", [(13, 10000), (37, 10000), (24, 300), (9, 1)])
+def test_edge_resolutions_and_step_caps_vs_c_restatement(torus, res, nsteps):
+    """Odd resolutions (the centre pixel's ray is NaN, as the reference's),
+    resolutions that leave partial 8x8 tiles, and step caps that stop rays
+    mid-flight (row N is never emitted) against the C restatement."""
+    p = images.render_params(0.5, camera_inclination=30, fov=20, resolution=res, max_nsteps=nsteps)
+    img, steps = images.render(torus.device_snapshot, p, return_steps=True)
+    cs = helpers.c_snapshot_from_model(torus)
+    ref, ref_steps = cs.render(0.5, 30, 1000.0, 20.0, res, nsteps, 13. / 9, 40.0, 1e26, 6.2e9 * MSUN, 230e9)
+    ref = ref.reshape(res, res)
+    np.testing.assert_array_equal(np.isfinite(img), np.isfinite(ref))
+    assert np.array_equal(steps.ravel(), ref_steps)
+    if ref.max() > 0:
+        rel, mask = _pixel_close(img, ref)
+        assert np.mean(rel < 1e-4) >= 0.99
+        assert abs(img.sum() - ref.sum()) <= 1e-5 * ref.sum()
+    else:
+        assert np.all(img[np.isfinite(img)] == 0)
+
+
+def test_image_outside_the_domain_is_zero(torus):
+    """A field of view that never meets the snapshot: every sample is outside
+    every block, and the image is exactly zero."""
+    p = images.render_params(0.5, camera_inclination=30, fov=2000, resolution=16)
+    img = images.render(torus.device_snapshot, p)
+    corners = img[[0, 0, -1, -1], [0, -1, 0, -1]]
+    assert np.all(corners == 0.0)
+    cs = helpers.c_snapshot_from_model(torus)
+    ref, _ = cs.render(0.5, 30, 1000.0, 2000.0, 16, 10000, 13. / 9, 40.0, 1e26, 6.2e9 * MSUN, 230e9)
+    np.testing.assert_allclose(img.ravel(), ref, rtol=1e-4, atol=0)

@@ -30,7 +30,7 @@ MIX = [(48, 0.5, 30.0, 230e9, 0), (32, -0.7, 75.0, 86e9, 1), (64, 0.9, 10.0, 345
 
 @pytest.fixture(scope="module")
 def models():
-    torus = AthenakFluidModel.from_datasets(syn.config2_snapshot(0.5, seed=0), 0.5, fluid_gamma=13. / 9)
+    torus = AthenakFluidModel.from_datasets(syn.uniform_torus_snapshot(0.5, seed=0), 0.5, fluid_gamma=13. / 9)
     small = AthenakFluidModel.from_datasets(helpers.small_two_level(n=4, seed=3), 0.5, fluid_gamma=13. / 9)
     return torus, small
 
