@@ -173,8 +173,15 @@ typedef struct mk_render_params {
   int32_t resolution;         /* pixels per side                            */
   int32_t max_nsteps;         /* N (10000)                                  */
   int32_t precision;          /* 0 = plain FP64, 1 = mixed FP32/FP64        */
-  int32_t flags;              /* reserved, 0                                */
+  int32_t flags;              /* MK_RENDER_* bits, 0 by default             */
 } mk_render_params;
+
+/* mk_render_params.flags: kernel form.  By default the library runs a small
+ * launch (at most one 8x8 tile per SIMD) in the paired form -- a geodesic
+ * wave and a transfer wave per tile -- and a larger one in the classic form
+ * (one wave per tile).  The bits force one form; both give the same image. */
+#define MK_RENDER_CLASSIC 1
+#define MK_RENDER_PAIRED 2
 
 /* make_image  (images.py:30-144) for a device-resident snapshot: camera,
  * geodesic, sampling, emission and transfer fused per ray; d_image

@@ -176,6 +176,195 @@ __global__ void __launch_bounds__(256) k_render(SnapView sv, RenderDev rp, doubl
   if (steps) steps[p] = n;
 }
 
+// --------------------------------------------------------------- paired form
+// For launches of at most one tile per SIMD (a 256^2 image is 1024 tiles on
+// 1024 SIMDs) the time is the slowest tile's step count times the step
+// latency of a wave alone on its SIMD.  The paired form gives each tile two
+// waves: a geodesic wave (camera ray, RK4, step rule) and a transfer wave
+// (block lookup, gather, interpolation, emission, transfer).  They run
+// concurrently and hand samples over through a ring of kPairDepth slots in
+// LDS, synchronised by two counters per pair (rounds produced, rounds
+// consumed) with release/acquire ordering at workgroup scope; no barrier
+// couples different tiles.  Waits are bounded: after kSpinLimit polls a wave
+// gives up and writes NaN for its pixels, so every wave exits.  The transfer
+// wave adds the samples in step order with the same expressions as the
+// classic form.
+constexpr int kPairTiles = 4;   // tiles per workgroup (geodesic + transfer wave each)
+constexpr int kPairDepth = 4;   // ring slots per tile
+constexpr int kPairComp = 9;    // state[8], dt
+constexpr size_t kPairRingDoubles = (size_t)kPairTiles * kPairDepth * kPairComp * 64;
+constexpr size_t kPairLdsExtra = kPairRingDoubles * sizeof(double) + kPairTiles * 4 * sizeof(int);
+constexpr int kSpinLimit = 1 << 24;
+
+// Polls *cnt (LDS) until it reaches target; false after kSpinLimit polls.
+__device__ __forceinline__ bool wait_at_least(int* cnt, int target) {
+  for (int spin = 0; spin < kSpinLimit; ++spin) {
+    const int v = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+    if (v >= target) return true;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return false;
+}
+
+__device__ __forceinline__ void publish(int* cnt, int v) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  if ((threadIdx.x & 63) == 0) __hip_atomic_store(cnt, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(128 * kPairTiles) k_render_pair(SnapView sv, RenderDev rp, double* __restrict__ img,
+                                                                  int32_t* __restrict__ steps, int tab_doubles) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  extern __shared__ double lds[];
+  double* ring_all = lds + tab_doubles;
+  int* ctr = reinterpret_cast<int*>(ring_all + kPairRingDoubles);  // per tile: produced, consumed, done, pad
+  if (threadIdx.x < 4 * kPairTiles) ctr[threadIdx.x] = ((threadIdx.x & 3) == 2) ? -1 : 0;
+  const double* meta;
+  const int* grid;
+  stage_tables<true>(sv, lds, meta, grid);  // ends with a workgroup barrier
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int pair = wave % kPairTiles;
+  const bool geo = wave < kPairTiles;
+  double* ring = ring_all + (size_t)pair * kPairDepth * kPairComp * 64 + lane;
+  int* produced = ctr + 4 * pair;
+  int* consumed = produced + 1;
+  int* done = produced + 2;
+  const unsigned nwg = gridDim.x, wg = blockIdx.x, per = nwg / 8;
+  const unsigned lwg = (nwg % 8 == 0) ? (wg % 8) * per + wg / 8 : wg;
+  const long tile = (long)lwg * kPairTiles + pair;
+  int pi = 0, pj = 0;
+  bool valid = tile < rp.ntiles;
+  if (valid) {
+    pi = (int)(tile / rp.tiles_x) * 8 + (lane >> 3);
+    pj = (int)(tile % rp.tiles_x) * 8 + (lane & 7);
+    valid = pi < rp.res && pj < rp.res;
+  }
+  const long p = (long)pi * rp.res + pj;
+  if (geo) {
+    double s[8], r = 0.0, dt = 0.0, drdl = 1e300;
+    int k = 0, nacc = 0;
+    bool alive = false;
+    if (valid) {
+      double sx[4], sv4[4];
+      camera_ray(grid_coord(rp.lo, rp.step, pi), grid_coord(rp.lo, rp.step, pj), rp.ci, rp.si, rp.dist, sx, sv4);
+      nullify(sx, sv4, rp.a, s);
+      r = radius_cal(s[1], s[2], s[3], rp.a);
+      dt = step_rule(r, rp.rEH, rp.div, rp.tol);
+      alive = true;
+    }
+    const bool zone = MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0);
+    bool ok = true;
+    int t = 0;
+    while (__any(alive)) {
+      if (!wait_at_least(consumed, t - kPairDepth + 1)) {
+        ok = false;
+        break;
+      }
+      double* out = ring + (size_t)(t % kPairDepth) * kPairComp * 64;
+      bool wrote = false;
+      if (alive) {
+        if (k >= rp.N || dt == 0.0) {
+          alive = false;
+        } else {
+          double nw[8];
+          if constexpr (!MIXED) {
+            rk4_step<double>(s, dt, rp.a, nw);
+          } else {
+            const bool fp64 = zone && ((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
+            if (fp64) rk4_step<double>(s, dt, rp.a, nw);
+            else rk4_step_mixed(s, dt, (float)rp.a, nw);
+          }
+          double rn, dtn;
+          if constexpr (!MIXED) {
+            rn = radius_cal(nw[1], nw[2], nw[3], rp.a);
+            dtn = step_rule(rn, rp.rEH, rp.div, rp.tol);
+          } else {
+            rn = radius_f32(nw[1], nw[2], nw[3], rp.a);
+            dtn = step_rule_fast(rn, rp.rEH, rp.inv_div, rp.div, rp.tol);
+          }
+          if (dtn == 0.0) {
+            alive = false;
+          } else {
+            ++nacc;
+            if (k + 1 < rp.N) {  // row N is never emitted
+#pragma unroll
+              for (int c = 0; c < 8; ++c) out[c * 64] = nw[c];
+              out[8 * 64] = dt;
+              wrote = true;
+            }
+            if (zone) drdl = (rn - r) / dt;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) s[c] = nw[c];
+            dt = dtn;
+            r = rn;
+            if (++k >= rp.N) alive = false;
+          }
+        }
+      }
+      if (!wrote) out[8 * 64] = 0.0;
+      publish(produced, ++t);
+    }
+    publish(done, ok ? t : -2);
+    if (valid && steps) steps[p] = ok ? nacc : -1;
+    return;
+  }
+  // transfer wave
+  double I = 0.0, trans = 1.0, ps[8], pdt = 0.0;
+  bool pend = false, ok = true;
+  Stencil pst;
+  Corners pc;
+  auto consume = [&]() {
+    if (pst.mb < 0) return;
+    T P[8], j, al;
+    interp<T>(pst, pc, P);
+    if (P[0] == T(0)) return;
+    emission_from_prims<T>(rp.ip, P, ps, rp.a, j, al);
+    const double c = -pdt * rp.ip.L_unit;
+    I += trans * (c * (double)j);
+    trans *= (1.0 - c * (double)al);
+  };
+  for (int t = 0;; ++t) {
+    // round t is ready, or the geodesic wave has finished after fewer rounds
+    int state = 0;
+    for (int spin = 0; spin < kSpinLimit; ++spin) {
+      const int pr = __builtin_amdgcn_readfirstlane(
+          __hip_atomic_load(produced, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+      if (pr > t) {
+        state = 1;
+        break;
+      }
+      const int dn = __builtin_amdgcn_readfirstlane(
+          __hip_atomic_load(done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+      if (dn == -2 || (dn >= 0 && t >= dn)) {
+        state = dn == -2 ? -1 : 2;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (state != 1) {
+      ok = state == 2;
+      break;
+    }
+    if (pend) {
+      consume();
+      pend = false;
+    }
+    const double* in = ring + (size_t)(t % kPairDepth) * kPairComp * 64;
+    pdt = in[8 * 64];
+    if (pdt != 0.0) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) ps[c] = in[c * 64];
+      pst = locate(sv, meta, grid, ps[1], ps[2], ps[3]);
+      fetch(sv, pst, pc);
+      pend = true;
+    }
+    publish(consumed, t + 1);  // the slot is in registers now
+  }
+  if (pend) consume();
+  if (valid) img[p] = ok ? I : __builtin_nan("");
+}
+
 size_t table_bytes(const SnapView& sv) {
   return (size_t)sv.nmb * kMeta * sizeof(double) + (size_t)sv.gx * sv.gy * sv.gz * sizeof(int);
 }
@@ -236,6 +425,31 @@ extern "C" int mk_render(const mk_snapshot* snap, const mk_render_params* p, dou
   const bool use_lds = lds <= kMaxTableLds;
   const dim3 grid((unsigned)nwg), block(64 * waves_per_wg);
   hipStream_t st = as_stream(stream);
+  // paired form: tables in LDS next to the rings, and at most one tile per SIMD
+  const size_t tab = (lds + 15) / 16 * 16;
+  const bool pair_fits = use_lds && tab + kPairLdsExtra <= 160 * 1024;
+  bool paired = false;
+  if (pair_fits && !(p->flags & MK_RENDER_CLASSIC)) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+      set_error("mk_render: no device");
+      return MK_ERR_HIP;
+    }
+    paired = (p->flags & MK_RENDER_PAIRED) || rp.ntiles <= 4L * cus;
+  }
+  if (paired) {
+    long nwg2 = (rp.ntiles + kPairTiles - 1) / kPairTiles;
+    nwg2 = (nwg2 + 7) / 8 * 8;
+    const size_t lds2 = tab + kPairLdsExtra;
+    const int tabd = (int)(tab / sizeof(double));
+    if (p->precision == 0)
+      k_render_pair<false><<<dim3((unsigned)nwg2), dim3(128 * kPairTiles), lds2, st>>>(sv, rp, d_image, d_steps, tabd);
+    else
+      k_render_pair<true><<<dim3((unsigned)nwg2), dim3(128 * kPairTiles), lds2, st>>>(sv, rp, d_image, d_steps, tabd);
+    MK_LAUNCH_CHECK();
+    return MK_OK;
+  }
   if (p->precision == 0) {
     if (use_lds) k_render<false, true><<<grid, block, lds, st>>>(sv, rp, d_image, d_steps);
     else k_render<false, false><<<grid, block, 0, st>>>(sv, rp, d_image, d_steps);

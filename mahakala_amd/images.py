@@ -16,6 +16,7 @@ from mahakala_amd import _lib
 from mahakala_amd.constants import Msun
 from mahakala_amd.electrons import rlow_rhigh_model
 
+FORMS = {None: 0, "classic": 1, "paired": 2}  # MK_RENDER_CLASSIC, MK_RENDER_PAIRED
 PRECISIONS = {"plain64": 0, "fp64": 0, "mixed": 1, "mixed-zone": 1}
 
 # Mixed precision keeps in FP64 the ray state and its RK4 accumulation, the
@@ -31,9 +32,14 @@ ZONES = {"mixed": (0.0, 0.0, 0.0), "mixed-zone": (4.0, 5.0, 0.1)}
 def render_params(bhspin, camera_inclination=60, camera_distance=1000, mass_scale=1.e26, M_bh=6.2e9 * Msun,
                   r_high=40, observing_frequency=230.e9, fov=20, resolution=160, max_nsteps=10000,
                   fluid_gamma=13. / 9, precision="plain64", fp64_radius=None, ring_radius=None,
-                  ring_drdl=None):
+                  ring_drdl=None, form=None):
+    """mk_render parameters.  form: None (the library chooses), "classic"
+    (one wave per 8x8 tile) or "paired" (a geodesic wave and a transfer wave
+    per tile); both forms give the same image."""
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+    if form not in FORMS:
+        raise ValueError(f"form must be one of {sorted(FORMS, key=str)}")
     p = _lib.mk_render_params()
     p.bhspin = float(bhspin)
     p.inclination = float(camera_inclination)
@@ -55,7 +61,7 @@ def render_params(bhspin, camera_inclination=60, camera_distance=1000, mass_scal
     p.resolution = int(resolution)
     p.max_nsteps = int(max_nsteps)
     p.precision = PRECISIONS[precision]
-    p.flags = 0
+    p.flags = FORMS[form]
     return p
 
 

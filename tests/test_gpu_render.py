@@ -160,3 +160,23 @@ def test_image_outside_the_domain_is_zero(torus):
     cs = helpers.c_snapshot_from_model(torus)
     ref, _ = cs.render(0.5, 30, 1000.0, 2000.0, 16, 10000, 13. / 9, 40.0, 1e26, 6.2e9 * MSUN, 230e9)
     np.testing.assert_allclose(img.ravel(), ref, rtol=1e-4, atol=0)
+
+
+@pytest.mark.parametrize("precision", ["plain64", "mixed", "mixed-zone"])
+@pytest.mark.parametrize("res", [24, 256])
+def test_paired_and_classic_forms_agree(torus, precision, res):
+    """The paired form (a geodesic wave and a transfer wave per tile, samples
+    handed over through LDS) and the classic form give the same image and the
+    same step counts."""
+    out = {}
+    for form in ("classic", "paired"):
+        p = images.render_params(0.5, camera_inclination=30, resolution=res, precision=precision, form=form)
+        out[form] = images.render(torus.device_snapshot, p, return_steps=True)
+    (a, sa), (b, sb) = out["classic"], out["paired"]
+    assert np.all(np.isfinite(b))
+    np.testing.assert_array_equal(sa, sb)
+    live = a > 1e-6 * a.max()
+    rel = np.abs(a - b)[live] / a[live]
+    print(f"[render] {precision} {res}^2 paired vs classic: max rel {rel.max():.2e}")
+    assert rel.max() < (1e-12 if precision == "plain64" else 1e-4)
+    assert abs(a.sum() - b.sum()) <= 1e-6 * a.sum()
