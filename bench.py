@@ -273,6 +273,13 @@ def main():
     par = {p: images.render_params(a, camera_inclination=inc, fov=fov, resolution=res, observing_frequency=nu,
                                    precision=p) for p in precisions}
 
+    def kernel_name(p):
+        import ctypes as C
+        form = C.c_int32()
+        L.check(L.lib().mk_render_form(snap.handle, par[p], C.byref(form)))
+        m = "true" if par[p].precision == 1 else "false"
+        return f"k_render_pair<{m}>" if form.value == images.FORMS["paired"] else f"k_render<{m}, true>"
+
     def launch(p, steps_out=False):
         def fn():
             L.check(L.lib().mk_render(snap.handle, par[p], img.ptr, stp.ptr if steps_out else None, None))
@@ -319,7 +326,7 @@ def main():
     peak = PEAK_FP32 if head == "mixed" else PEAK_FP64
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
                     frac=round(achieved / peak, 4), traffic=None,
-                    kernel=f"k_render<{'true' if head == 'mixed' else 'false'}>",
+                    kernel=kernel_name(head),
                     algorithmic=f"{FLOP_PER_STEP:.0f} FLOP x {total_steps:.0f} accepted ray-steps per launch")
     tfile = os.path.join(ROOT, "profiles", "render_traffic.json")
     if os.path.exists(tfile):

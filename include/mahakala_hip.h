@@ -177,9 +177,10 @@ typedef struct mk_render_params {
 } mk_render_params;
 
 /* mk_render_params.flags: kernel form.  By default the library runs a small
- * launch (at most one 8x8 tile per SIMD) in the paired form -- a geodesic
- * wave and a transfer wave per tile -- and a larger one in the classic form
- * (one wave per tile).  The bits force one form; both give the same image. */
+ * launch (up to about 1.75 8x8 tiles per SIMD) in the paired form -- a
+ * geodesic wave and a transfer wave per tile -- and a larger one in the
+ * classic form (one wave per tile).  The bits force one form; both give the
+ * same image. */
 #define MK_RENDER_CLASSIC 1
 #define MK_RENDER_PAIRED 2
 
@@ -188,6 +189,9 @@ typedef struct mk_render_params {
  * (res*res,) FP64, d_steps (res*res,) accepted steps per ray (may be NULL). */
 int mk_render(const mk_snapshot* snap, const mk_render_params* params, double* d_image, int32_t* d_steps,
               void* stream);
+/* the kernel form mk_render runs for these parameters -> *h_form
+ * (MK_RENDER_CLASSIC or MK_RENDER_PAIRED); no reference counterpart */
+int mk_render_form(const mk_snapshot* snap, const mk_render_params* params, int32_t* h_form);
 
 /* ---------------------------------------------------------------- serving */
 /* Continuous batching of image and shadow requests on one GPU.  No single

@@ -115,6 +115,7 @@ _SIGS = {
     "mk_sample_prims": [vp, vp, lng, vp, vp],
     "mk_sample_scalars": [vp, vp, lng, dbl, dbl, vp, vp],
     "mk_render": [vp, C.POINTER(mk_render_params), vp, vp, vp],
+    "mk_render_form": [vp, C.POINTER(mk_render_params), C.POINTER(i32)],
     "mk_server_create": [C.POINTER(mk_server_config), C.POINTER(vp)],
     "mk_server_add_snapshot": [vp, vp, C.POINTER(i32)],
     "mk_server_submit_image": [vp, C.POINTER(mk_image_request), vp, C.POINTER(C.c_int64)],

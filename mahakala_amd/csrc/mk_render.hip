@@ -411,6 +411,40 @@ int fill_render_dev(const mk_render_params* p, RenderDev* rp) {
 
 using namespace mk;
 
+// Paired form where the tables fit in LDS next to the rings and the launch
+// has fewer than about 1.75 tiles per SIMD (measured on MI355X: paired is
+// 15-24 % faster up to 1600 tiles on 1024 SIMDs, and 8-24 % slower from 2304
+// tiles, where three classic workgroups per CU keep more waves in flight).
+static int choose_form(const SnapView& sv, const RenderDev& rp, int flags, int* form) {
+  const size_t tab = (table_bytes(sv) + 15) / 16 * 16;
+  const bool fits = table_bytes(sv) <= kMaxTableLds && tab + kPairLdsExtra <= 160 * 1024;
+  *form = MK_RENDER_CLASSIC;
+  if (!fits || (flags & MK_RENDER_CLASSIC)) return MK_OK;
+  if (flags & MK_RENDER_PAIRED) {
+    *form = MK_RENDER_PAIRED;
+    return MK_OK;
+  }
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    set_error("mk_render: no device");
+    return MK_ERR_HIP;
+  }
+  if (rp.ntiles * 4 <= 7L * 4 * cus) *form = MK_RENDER_PAIRED;
+  return MK_OK;
+}
+
+extern "C" int mk_render_form(const mk_snapshot* snap, const mk_render_params* p, int32_t* h_form) {
+  MK_REQUIRE(snap && p && h_form, "mk_render_form: null argument");
+  RenderDev rp;
+  int rc = fill_render_dev(p, &rp);
+  if (rc) return rc;
+  int form = 0;
+  rc = choose_form(snap->view(), rp, p->flags, &form);
+  *h_form = form;
+  return rc;
+}
+
 extern "C" int mk_render(const mk_snapshot* snap, const mk_render_params* p, double* d_image, int32_t* d_steps,
                          void* stream) {
   MK_REQUIRE(snap && p && d_image, "mk_render: null argument");
@@ -425,20 +459,11 @@ extern "C" int mk_render(const mk_snapshot* snap, const mk_render_params* p, dou
   const bool use_lds = lds <= kMaxTableLds;
   const dim3 grid((unsigned)nwg), block(64 * waves_per_wg);
   hipStream_t st = as_stream(stream);
-  // paired form: tables in LDS next to the rings, and at most one tile per SIMD
   const size_t tab = (lds + 15) / 16 * 16;
-  const bool pair_fits = use_lds && tab + kPairLdsExtra <= 160 * 1024;
-  bool paired = false;
-  if (pair_fits && !(p->flags & MK_RENDER_CLASSIC)) {
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
-      set_error("mk_render: no device");
-      return MK_ERR_HIP;
-    }
-    paired = (p->flags & MK_RENDER_PAIRED) || rp.ntiles <= 4L * cus;
-  }
-  if (paired) {
+  int form = 0;
+  rc = choose_form(sv, rp, p->flags, &form);
+  if (rc) return rc;
+  if (form == MK_RENDER_PAIRED) {
     long nwg2 = (rp.ntiles + kPairTiles - 1) / kPairTiles;
     nwg2 = (nwg2 + 7) / 8 * 8;
     const size_t lds2 = tab + kPairLdsExtra;
