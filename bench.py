@@ -237,6 +237,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-heavy", action="store_true", help="skip the heavy queued stream (8 snapshots)")
     ap.add_argument("--no-target", action="store_true", help="skip the a=0.9 precision-target image")
+    ap.add_argument("--form", choices=["auto", "classic", "paired"], default="auto",
+                    help="render kernel form (auto: the library's choice)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the device copy kernel over MIB MiB 3 times (traffic counter calibration)")
     args = ap.parse_args()
@@ -270,8 +272,9 @@ def main():
     img = L.DeviceBuffer(8 * n)
     stp = L.DeviceBuffer(4 * n)
     precisions = ("plain64", "mixed", "mixed-zone")
+    form = None if args.form == "auto" else args.form
     par = {p: images.render_params(a, camera_inclination=inc, fov=fov, resolution=res, observing_frequency=nu,
-                                   precision=p) for p in precisions}
+                                   precision=p, form=form) for p in precisions}
 
     def kernel_name(p):
         import ctypes as C
