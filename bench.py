@@ -237,6 +237,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-heavy", action="store_true", help="skip the heavy queued stream (8 snapshots)")
     ap.add_argument("--no-target", action="store_true", help="skip the a=0.9 precision-target image")
+    ap.add_argument("--no-sizes", action="store_true", help="skip the 512^2 and 1024^2 images")
     ap.add_argument("--form", choices=["auto", "classic", "paired"], default="auto",
                     help="render kernel form (auto: the library's choice)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -369,6 +370,24 @@ def main():
                                  "speedup_over_plain": round(pw / zw, 3),
                                  "accuracy_vs_plain": {k: (round(v, 9) if isinstance(v, float) else v)
                                                        for k, v in accuracy(I_mix["mixed-zone"]).items()}}
+
+    if not args.no_sizes:
+        # the same image at larger resolutions: more tiles than SIMDs, so the
+        # launch is no longer bound by its longest ray
+        sizes = {}
+        for r2 in (512, 1024):
+            img2 = L.DeviceBuffer(8 * r2 * r2)
+            p2 = images.render_params(a, camera_inclination=inc, fov=fov, resolution=r2, observing_frequency=nu,
+                                      precision=head if head != "mixed-zone" else "mixed")
+
+            def fn2(p2=p2, img2=img2):
+                L.check(L.lib().mk_render(snap.handle, p2, img2.ptr, None, None))
+            ms2, w2 = time_launches(L, fn2, max(3, args.steps // 5), 2)
+            sizes[f"{r2}x{r2}"] = {"ms_per_image": round(float(np.mean(ms2)), 3),
+                                   "rays_per_s": round(r2 * r2 / (float(np.mean(ms2)) * 1e-3), 1)}
+            del img2
+        out["larger_images"] = dict(sizes, what="same scene and precision as value, one launch per image")
+        print(f"[bench] larger images: {sizes}", file=sys.stderr, flush=True)
 
     if not args.no_target:
         # the precision target's case: a = 0.9, 256^2, i = 60 deg (make_image's
