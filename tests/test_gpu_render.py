@@ -180,3 +180,21 @@ def test_paired_and_classic_forms_agree(torus, precision, res):
     print(f"[render] {precision} {res}^2 paired vs classic: max rel {rel.max():.2e}")
     assert rel.max() < (1e-12 if precision == "plain64" else 1e-4)
     assert abs(a.sum() - b.sum()) <= 1e-6 * a.sum()
+
+
+@pytest.mark.parametrize("precision", ["plain64", "mixed"])
+def test_face_on_image_is_invariant_under_quarter_turns(torus, precision):
+    """At inclination 0 the camera looks down the spin axis; the torus, its
+    flow and field, the Cartesian block grid and the pixel grid are all
+    invariant under a quarter turn about that axis, so the full 256^2 image is
+    too (up to the rounding of different arithmetic orders)."""
+    p = images.render_params(0.5, camera_inclination=0, fov=20, resolution=256, precision=precision)
+    img = images.render(torus.device_snapshot, p)
+    assert img.max() > 0
+    live = img > 1e-6 * img.max()
+    for k in (1, 2, 3):
+        rot = np.rot90(img, k)
+        rel = np.abs(rot - img)[live] / img[live]
+        print(f"[render] {precision} face-on quarter turn {k}: max rel {rel.max():.2e}")
+        assert np.median(rel) < (1e-10 if precision == "plain64" else 1e-5)
+        assert abs(rot.sum() - img.sum()) <= (1e-10 if precision == "plain64" else 1e-6) * img.sum()
