@@ -238,7 +238,7 @@ def main():
     ap.add_argument("--no-heavy", action="store_true", help="skip the heavy queued stream (8 snapshots)")
     ap.add_argument("--no-target", action="store_true", help="skip the a=0.9 precision-target image")
     ap.add_argument("--no-sizes", action="store_true", help="skip the 512^2 and 1024^2 images")
-    ap.add_argument("--form", choices=["auto", "classic", "paired"], default="auto",
+    ap.add_argument("--form", choices=["auto", "classic", "paired", "triple"], default="auto",
                     help="render kernel form (auto: the library's choice)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the device copy kernel over MIB MiB 3 times (traffic counter calibration)")
@@ -282,6 +282,8 @@ def main():
         form = C.c_int32()
         L.check(L.lib().mk_render_form(snap.handle, par[p], C.byref(form)))
         m = "true" if par[p].precision == 1 else "false"
+        if form.value == images.FORMS["triple"]:
+            return f"k_render_tri<{m}>"
         return f"k_render_pair<{m}>" if form.value == images.FORMS["paired"] else f"k_render<{m}, true>"
 
     def launch(p, steps_out=False):
@@ -336,7 +338,7 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             tr = json.load(f).get(head)
-        if tr:
+        if tr and tr.get("kernel") == roofline["kernel"]:
             roofline["traffic"] = tr["bytes_per_launch"]
 
     out = {

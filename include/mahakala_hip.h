@@ -177,12 +177,16 @@ typedef struct mk_render_params {
 } mk_render_params;
 
 /* mk_render_params.flags: kernel form.  By default the library runs a small
- * launch (up to about 1.75 8x8 tiles per SIMD) in the paired form -- a
- * geodesic wave and a transfer wave per tile -- and a larger one in the
- * classic form (one wave per tile).  The bits force one form; both give the
- * same image. */
+ * launch (up to about 1.75 8x8 tiles per SIMD) in the triple form -- a
+ * geodesic wave, a gather wave (block lookup, cell loads, interpolation) and
+ * an emission wave per tile -- and a larger one in the classic form (one
+ * wave per tile).  The paired form runs a geodesic wave and a transfer wave
+ * per tile.  The bits force one form (the triple form falls back to the
+ * paired one where its LDS rings do not fit); all give the same image, bit
+ * for bit. */
 #define MK_RENDER_CLASSIC 1
 #define MK_RENDER_PAIRED 2
+#define MK_RENDER_TRIPLE 4
 
 /* make_image  (images.py:30-144) for a device-resident snapshot: camera,
  * geodesic, sampling, emission and transfer fused per ray; d_image
@@ -190,7 +194,8 @@ typedef struct mk_render_params {
 int mk_render(const mk_snapshot* snap, const mk_render_params* params, double* d_image, int32_t* d_steps,
               void* stream);
 /* the kernel form mk_render runs for these parameters -> *h_form
- * (MK_RENDER_CLASSIC or MK_RENDER_PAIRED); no reference counterpart */
+ * (MK_RENDER_CLASSIC, MK_RENDER_PAIRED or MK_RENDER_TRIPLE); no reference
+ * counterpart */
 int mk_render_form(const mk_snapshot* snap, const mk_render_params* params, int32_t* h_form);
 
 /* ---------------------------------------------------------------- serving */

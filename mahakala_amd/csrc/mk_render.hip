@@ -365,6 +365,218 @@ __global__ void __launch_bounds__(128 * kPairTiles) k_render_pair(SnapView sv, R
   if (valid) img[p] = ok ? I : __builtin_nan("");
 }
 
+// ------------------------------------------------------------- triple form
+// Three waves per tile on one SIMD -- geodesic, gather (lookup, cell loads,
+// interpolation) and emission (fluid scalars, j and alpha, transfer) --
+// chained by two LDS rings with the counters of the paired form: three
+// independent instruction streams fill the SIMD better than two.  Ring A
+// carries (state, dt) from the geodesic wave, ring B the eight interpolated
+// primitives and the state in the emission's precision (FP32 values are what
+// the FP32 emission would round them to) and dt in FP64 (0: nothing to add).
+constexpr int kTriDepth = 2;
+constexpr int kTriCompB = 15;  // P[8], s[1..7]
+constexpr size_t kTriRingA = (size_t)kPairTiles * kTriDepth * kPairComp * 64;  // doubles
+constexpr size_t kTriRingBd = (size_t)kPairTiles * kTriDepth * 64;              // doubles (dt)
+constexpr size_t kTriRingBt = (size_t)kPairTiles * kTriDepth * kTriCompB * 64;  // elements of T
+
+template <bool MIXED>
+constexpr size_t tri_lds_extra() {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  return (kTriRingA + kTriRingBd) * sizeof(double) + kTriRingBt * sizeof(T) + kPairTiles * 8 * sizeof(int);
+}
+
+// Waits for round t of a ring: 1 ready, 0 the writer finished after fewer
+// rounds, -1 failure (the writer gave up, or kSpinLimit polls).
+__device__ __forceinline__ int wait_round(int* produced, int* done, int t) {
+  for (int spin = 0; spin < kSpinLimit; ++spin) {
+    const int pr = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(produced, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+    if (pr > t) return 1;
+    const int dn = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+    if (dn == -2) return -1;
+    if (dn >= 0 && t >= dn) return 0;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return -1;
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, RenderDev rp, double* __restrict__ img,
+                                                                  int32_t* __restrict__ steps, int tab_doubles) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  extern __shared__ double lds[];
+  double* ringA_all = lds + tab_doubles;
+  double* ringBd_all = ringA_all + kTriRingA;
+  T* ringBt_all = reinterpret_cast<T*>(ringBd_all + kTriRingBd);
+  // per tile: producedA, consumedA, doneA, producedB, consumedB, doneB, pad, pad
+  int* ctr = reinterpret_cast<int*>(ringBt_all + kTriRingBt);
+  if (threadIdx.x < 8 * kPairTiles) {
+    const int f = threadIdx.x & 7;
+    ctr[threadIdx.x] = (f == 2 || f == 5) ? -1 : 0;
+  }
+  const double* meta;
+  const int* grid;
+  stage_tables<true>(sv, lds, meta, grid);  // ends with a workgroup barrier
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // tile k's waves are k, k + 4 and k + 8: the hardware places them on one SIMD
+  const int tl = wave % kPairTiles, stage = wave / kPairTiles;
+  double* ringA = ringA_all + (size_t)tl * kTriDepth * kPairComp * 64 + lane;
+  double* ringBd = ringBd_all + (size_t)tl * kTriDepth * 64 + lane;
+  T* ringBt = ringBt_all + (size_t)tl * kTriDepth * kTriCompB * 64 + lane;
+  int* c = ctr + 8 * tl;
+  const unsigned nwg = gridDim.x, wg = blockIdx.x, per = nwg / 8;
+  const unsigned lwg = (nwg % 8 == 0) ? (wg % 8) * per + wg / 8 : wg;
+  const long tile = (long)lwg * kPairTiles + tl;
+  int pi = 0, pj = 0;
+  bool valid = tile < rp.ntiles;
+  if (valid) {
+    pi = (int)(tile / rp.tiles_x) * 8 + (lane >> 3);
+    pj = (int)(tile % rp.tiles_x) * 8 + (lane & 7);
+    valid = pi < rp.res && pj < rp.res;
+  }
+  const long p = (long)pi * rp.res + pj;
+  if (stage == 0) {  // geodesic
+    double s[8], r = 0.0, dt = 0.0, drdl = 1e300;
+    int k = 0, nacc = 0;
+    bool alive = false;
+    if (valid) {
+      double sx[4], sv4[4];
+      camera_ray(grid_coord(rp.lo, rp.step, pi), grid_coord(rp.lo, rp.step, pj), rp.ci, rp.si, rp.dist, sx, sv4);
+      nullify(sx, sv4, rp.a, s);
+      r = radius_cal(s[1], s[2], s[3], rp.a);
+      dt = step_rule(r, rp.rEH, rp.div, rp.tol);
+      alive = true;
+    }
+    const bool zone = MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0);
+    bool ok = true;
+    int t = 0;
+    while (__any(alive)) {
+      if (!wait_at_least(c + 1, t - kTriDepth + 1)) {
+        ok = false;
+        break;
+      }
+      double* out = ringA + (size_t)(t % kTriDepth) * kPairComp * 64;
+      bool wrote = false;
+      if (alive) {
+        if (k >= rp.N || dt == 0.0) {
+          alive = false;
+        } else {
+          double nw[8];
+          if constexpr (!MIXED) {
+            rk4_step<double>(s, dt, rp.a, nw);
+          } else {
+            const bool fp64 = zone && ((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
+            if (fp64) rk4_step<double>(s, dt, rp.a, nw);
+            else rk4_step_mixed(s, dt, (float)rp.a, nw);
+          }
+          double rn, dtn;
+          if constexpr (!MIXED) {
+            rn = radius_cal(nw[1], nw[2], nw[3], rp.a);
+            dtn = step_rule(rn, rp.rEH, rp.div, rp.tol);
+          } else {
+            rn = radius_f32(nw[1], nw[2], nw[3], rp.a);
+            dtn = step_rule_fast(rn, rp.rEH, rp.inv_div, rp.div, rp.tol);
+          }
+          if (dtn == 0.0) {
+            alive = false;
+          } else {
+            ++nacc;
+            if (k + 1 < rp.N) {  // row N is never emitted
+#pragma unroll
+              for (int q = 0; q < 8; ++q) out[q * 64] = nw[q];
+              out[8 * 64] = dt;
+              wrote = true;
+            }
+            if (zone) drdl = (rn - r) / dt;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s[q] = nw[q];
+            dt = dtn;
+            r = rn;
+            if (++k >= rp.N) alive = false;
+          }
+        }
+      }
+      if (!wrote) out[8 * 64] = 0.0;
+      publish(c + 0, ++t);
+    }
+    publish(c + 2, ok ? t : -2);
+    if (valid && steps) steps[p] = ok ? nacc : -1;
+    return;
+  }
+  if (stage == 1) {  // gather: lookup, cell loads, interpolation
+    bool ok = true;
+    int t = 0;
+    for (;; ++t) {
+      const int st = wait_round(c + 0, c + 2, t);
+      if (st <= 0) {
+        ok = st == 0;
+        break;
+      }
+      if (!wait_at_least(c + 4, t - kTriDepth + 1)) {
+        ok = false;
+        break;
+      }
+      const double* in = ringA + (size_t)(t % kTriDepth) * kPairComp * 64;
+      double pdt = in[8 * 64];
+      double ps[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) ps[q] = in[q * 64];
+      publish(c + 1, t + 1);  // ring A slot is in registers
+      T* ob = ringBt + (size_t)(t % kTriDepth) * kTriCompB * 64;
+      double* obd = ringBd + (size_t)(t % kTriDepth) * 64;
+      T P[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) P[q] = T(0);
+      if (pdt != 0.0) {
+        const Stencil stc = locate(sv, meta, grid, ps[1], ps[2], ps[3]);
+        if (stc.mb >= 0) {
+          Corners cr;
+          fetch(sv, stc, cr);
+          interp<T>(stc, cr, P);
+        }
+        if (stc.mb < 0 || P[0] == T(0)) pdt = 0.0;  // adds exactly nothing
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) ob[q * 64] = P[q];
+#pragma unroll
+      for (int q = 1; q < 8; ++q) ob[(7 + q) * 64] = (T)ps[q];
+      obd[0] = pdt;
+      publish(c + 3, t + 1);
+    }
+    publish(c + 5, ok ? t : -2);
+    return;
+  }
+  // emission and transfer, in step order
+  double I = 0.0, trans = 1.0;
+  bool ok = true;
+  for (int t = 0;; ++t) {
+    const int st = wait_round(c + 3, c + 5, t);
+    if (st <= 0) {
+      ok = st == 0;
+      break;
+    }
+    const T* ib = ringBt + (size_t)(t % kTriDepth) * kTriCompB * 64;
+    const double pdt = ringBd[(size_t)(t % kTriDepth) * 64];
+    T P[8];
+    double ps[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) P[q] = ib[q * 64];
+    ps[0] = 0.0;
+#pragma unroll
+    for (int q = 1; q < 8; ++q) ps[q] = (double)ib[(7 + q) * 64];
+    publish(c + 4, t + 1);  // ring B slot is in registers
+    if (pdt != 0.0) {
+      T j, al;
+      emission_from_prims<T>(rp.ip, P, ps, rp.a, j, al);
+      const double cc = -pdt * rp.ip.L_unit;
+      I += trans * (cc * (double)j);
+      trans *= (1.0 - cc * (double)al);
+    }
+  }
+  if (valid) img[p] = ok ? I : __builtin_nan("");
+}
+
 size_t table_bytes(const SnapView& sv) {
   return (size_t)sv.nmb * kMeta * sizeof(double) + (size_t)sv.gx * sv.gy * sv.gz * sizeof(int);
 }
@@ -415,11 +627,17 @@ using namespace mk;
 // has fewer than about 1.75 tiles per SIMD (measured on MI355X: paired is
 // 15-24 % faster up to 1600 tiles on 1024 SIMDs, and 8-24 % slower from 2304
 // tiles, where three classic workgroups per CU keep more waves in flight).
-static int choose_form(const SnapView& sv, const RenderDev& rp, int flags, int* form) {
+static int choose_form(const SnapView& sv, const RenderDev& rp, int precision, int flags, int* form) {
   const size_t tab = (table_bytes(sv) + 15) / 16 * 16;
   const bool fits = table_bytes(sv) <= kMaxTableLds && tab + kPairLdsExtra <= 160 * 1024;
+  const bool fits3 =
+      fits && tab + (precision == 1 ? tri_lds_extra<true>() : tri_lds_extra<false>()) <= 160 * 1024;
   *form = MK_RENDER_CLASSIC;
   if (!fits || (flags & MK_RENDER_CLASSIC)) return MK_OK;
+  if (flags & MK_RENDER_TRIPLE) {
+    *form = fits3 ? MK_RENDER_TRIPLE : MK_RENDER_PAIRED;
+    return MK_OK;
+  }
   if (flags & MK_RENDER_PAIRED) {
     *form = MK_RENDER_PAIRED;
     return MK_OK;
@@ -430,7 +648,7 @@ static int choose_form(const SnapView& sv, const RenderDev& rp, int flags, int* 
     set_error("mk_render: no device");
     return MK_ERR_HIP;
   }
-  if (rp.ntiles * 4 <= 7L * 4 * cus) *form = MK_RENDER_PAIRED;
+  if (rp.ntiles * 4 <= 7L * 4 * cus) *form = fits3 ? MK_RENDER_TRIPLE : MK_RENDER_PAIRED;
   return MK_OK;
 }
 
@@ -440,7 +658,7 @@ extern "C" int mk_render_form(const mk_snapshot* snap, const mk_render_params* p
   int rc = fill_render_dev(p, &rp);
   if (rc) return rc;
   int form = 0;
-  rc = choose_form(snap->view(), rp, p->flags, &form);
+  rc = choose_form(snap->view(), rp, p->precision, p->flags, &form);
   *h_form = form;
   return rc;
 }
@@ -461,8 +679,21 @@ extern "C" int mk_render(const mk_snapshot* snap, const mk_render_params* p, dou
   hipStream_t st = as_stream(stream);
   const size_t tab = (lds + 15) / 16 * 16;
   int form = 0;
-  rc = choose_form(sv, rp, p->flags, &form);
+  rc = choose_form(sv, rp, p->precision, p->flags, &form);
   if (rc) return rc;
+  if (form == MK_RENDER_TRIPLE) {
+    long nwg3 = (rp.ntiles + kPairTiles - 1) / kPairTiles;
+    nwg3 = (nwg3 + 7) / 8 * 8;
+    const int tabd = (int)(tab / sizeof(double));
+    if (p->precision == 0)
+      k_render_tri<false><<<dim3((unsigned)nwg3), dim3(192 * kPairTiles), tab + tri_lds_extra<false>(), st>>>(
+          sv, rp, d_image, d_steps, tabd);
+    else
+      k_render_tri<true><<<dim3((unsigned)nwg3), dim3(192 * kPairTiles), tab + tri_lds_extra<true>(), st>>>(
+          sv, rp, d_image, d_steps, tabd);
+    MK_LAUNCH_CHECK();
+    return MK_OK;
+  }
   if (form == MK_RENDER_PAIRED) {
     long nwg2 = (rp.ntiles + kPairTiles - 1) / kPairTiles;
     nwg2 = (nwg2 + 7) / 8 * 8;

@@ -16,7 +16,7 @@ from mahakala_amd import _lib
 from mahakala_amd.constants import Msun
 from mahakala_amd.electrons import rlow_rhigh_model
 
-FORMS = {None: 0, "classic": 1, "paired": 2}  # MK_RENDER_CLASSIC, MK_RENDER_PAIRED
+FORMS = {None: 0, "classic": 1, "paired": 2, "triple": 4}  # MK_RENDER_CLASSIC, _PAIRED, _TRIPLE
 PRECISIONS = {"plain64": 0, "fp64": 0, "mixed": 1, "mixed-zone": 1}
 
 # Mixed precision keeps in FP64 the ray state and its RK4 accumulation, the
@@ -34,8 +34,9 @@ def render_params(bhspin, camera_inclination=60, camera_distance=1000, mass_scal
                   fluid_gamma=13. / 9, precision="plain64", fp64_radius=None, ring_radius=None,
                   ring_drdl=None, form=None):
     """mk_render parameters.  form: None (the library chooses), "classic"
-    (one wave per 8x8 tile) or "paired" (a geodesic wave and a transfer wave
-    per tile); both forms give the same image."""
+    (one wave per 8x8 tile), "paired" (a geodesic wave and a transfer wave
+    per tile) or "triple" (geodesic, gather and emission waves per tile);
+    all forms give the same image, bit for bit."""
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
     if form not in FORMS:
