@@ -403,7 +403,8 @@ __device__ __forceinline__ int wait_round(int* produced, int* done, int t) {
 
 template <bool MIXED>
 __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, RenderDev rp, double* __restrict__ img,
-                                                                  int32_t* __restrict__ steps, int tab_doubles) {
+                                                                  int32_t* __restrict__ steps, int tab_doubles,
+                                                                  int strided) {
   using T = typename std::conditional<MIXED, float, double>::type;
   extern __shared__ double lds[];
   double* ringA_all = lds + tab_doubles;
@@ -419,15 +420,22 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   const int* grid;
   stage_tables<true>(sv, lds, meta, grid);  // ends with a workgroup barrier
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // tile k's waves are k, k + 4 and k + 8: the hardware places them on one SIMD
-  const int tl = wave % kPairTiles, stage = wave / kPairTiles;
+  // Wave w runs on SIMD w % 4 (the hardware's placement).  The three waves of
+  // tile k sit on SIMDs k, k + 1 and k + 2: every SIMD holds one wave of each
+  // stage, and a tile that outlives the others of its workgroup has its
+  // stages on three SIMDs instead of sharing one.
+  const int stage = wave / kPairTiles;
+  const int tl = (wave + (kPairTiles - 1) * stage) % kPairTiles;
   double* ringA = ringA_all + (size_t)tl * kTriDepth * kPairComp * 64 + lane;
   double* ringBd = ringBd_all + (size_t)tl * kTriDepth * 64 + lane;
   T* ringBt = ringBt_all + (size_t)tl * kTriDepth * kTriCompB * 64 + lane;
   int* c = ctr + 8 * tl;
   const unsigned nwg = gridDim.x, wg = blockIdx.x, per = nwg / 8;
   const unsigned lwg = (nwg % 8 == 0) ? (wg % 8) * per + wg / 8 : wg;
-  const long tile = (long)lwg * kPairTiles + tl;
+  // strided: the four tiles of a workgroup come from four bands of the image,
+  // so long tiles (rays near the photon ring) rarely share a CU; used when
+  // every workgroup is resident at once.  Otherwise adjacent tiles.
+  const long tile = strided ? (long)tl * nwg + lwg : (long)lwg * kPairTiles + tl;
   int pi = 0, pj = 0;
   bool valid = tile < rp.ntiles;
   if (valid) {
@@ -685,12 +693,27 @@ extern "C" int mk_render(const mk_snapshot* snap, const mk_render_params* p, dou
     long nwg3 = (rp.ntiles + kPairTiles - 1) / kPairTiles;
     nwg3 = (nwg3 + 7) / 8 * 8;
     const int tabd = (int)(tab / sizeof(double));
+    // strided tiles when every workgroup of the launch is resident at once
+    const size_t lds3 = tab + (p->precision == 0 ? tri_lds_extra<false>() : tri_lds_extra<true>());
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+      e = p->precision == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_tri<false>,
+                                                                           192 * kPairTiles, lds3)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_tri<true>,
+                                                                           192 * kPairTiles, lds3);
+    if (e != hipSuccess) {
+      set_error("mk_render: occupancy query failed");
+      return MK_ERR_HIP;
+    }
+    const int strided = nwg3 <= (long)per_cu * cus;
     if (p->precision == 0)
-      k_render_tri<false><<<dim3((unsigned)nwg3), dim3(192 * kPairTiles), tab + tri_lds_extra<false>(), st>>>(
-          sv, rp, d_image, d_steps, tabd);
+      k_render_tri<false><<<dim3((unsigned)nwg3), dim3(192 * kPairTiles), lds3, st>>>(sv, rp, d_image, d_steps, tabd,
+                                                                             strided);
     else
-      k_render_tri<true><<<dim3((unsigned)nwg3), dim3(192 * kPairTiles), tab + tri_lds_extra<true>(), st>>>(
-          sv, rp, d_image, d_steps, tabd);
+      k_render_tri<true><<<dim3((unsigned)nwg3), dim3(192 * kPairTiles), lds3, st>>>(sv, rp, d_image, d_steps, tabd,
+                                                                            strided);
     MK_LAUNCH_CHECK();
     return MK_OK;
   }
