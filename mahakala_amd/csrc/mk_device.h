@@ -475,11 +475,36 @@ __device__ __forceinline__ void fetch(const SnapView& sv, const Stencil& st, Cor
       }
 }
 
-// Interpolate x1, then x2, then x3 (athenak.py:746-752), whatever the
-// stencil (callers discard the result of a stencil outside the domain).
+// FP64: interpolate x1, then x2, then x3 (athenak.py:746-752).  FP32 (the
+// mixed path): the same trilinear interpolant as a weighted sum of the 8
+// corners (weights formed once, shared by the 8 primitives; 1 + 7 FMA per
+// primitive instead of 7 lerps).  Whatever the stencil: callers discard the
+// result of a stencil outside the domain.
 template <typename T>
 __device__ __forceinline__ void interp_raw(const Stencil& st, const Corners& c, T P[8]) {
   const T f0 = (T)st.fr[0], f1 = (T)st.fr[1], f2 = (T)st.fr[2];
+  if constexpr (sizeof(T) == 4) {
+    const T g0 = T(1) - f0, g1 = T(1) - f1, g2 = T(1) - f2;
+    const T wyz[4] = {g1 * g2, f1 * g2, g1 * f2, f1 * f2};  // index x3*2 + x2
+    T w[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      w[2 * q] = g0 * wyz[q];
+      w[2 * q + 1] = f0 * wyz[q];
+    }
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      T acc = T(0);
+#pragma unroll
+      for (int c8 = 0; c8 < 8; ++c8) {
+        const float4 e = c.q[c8][v >> 2];
+        const int r = v & 3;
+        acc += w[c8] * (r == 0 ? e.x : (r == 1 ? e.y : (r == 2 ? e.z : e.w)));
+      }
+      P[v] = acc;
+    }
+    return;
+  }
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
     T s[8];
@@ -571,7 +596,8 @@ template <typename T>
 __device__ __forceinline__ void synchrotron_sin(T Ne, T Th, T B, T sin_pitch, T nu, bool invariant, T rescale,
                                                 T& jo, T& ao) {
   using M = Math<T>;
-  const T nuc = T(kEE) * B / T(2.0 * kPI * kME * kCL);
+  // FP32: constant quotients folded into one product
+  const T nuc = sizeof(T) == 8 ? T(kEE) * B / T(2.0 * kPI * kME * kCL) : B * T(kEE / (2.0 * kPI * kME * kCL));
   const T nus = T(2.0 / 9.0) * nuc * (Th * Th) * sin_pitch;
   const T X = M::div(nu, nus);
   const T c3 = M::cbrt(X);
@@ -582,7 +608,7 @@ __device__ __forceinline__ void synchrotron_sin(T Ne, T Th, T B, T sin_pitch, T 
   if (X > T(1e12)) em = T(0);
   if (Th < T(0.3)) em = T(0);
   const T bx = M::div(T(kHPL) * nu, T(kME * kCL * kCL) * Th);
-  T Bden = bx / T(24) * (T(24) + bx * (T(12) + bx * (T(4) + bx)));
+  T Bden = (sizeof(T) == 8 ? bx / T(24) : bx * T(1.0 / 24)) * (T(24) + bx * (T(12) + bx * (T(4) + bx)));
   if (!(bx < T(2e-3))) Bden = M::exp(bx) - T(1);  // rarely taken: a branch, not a select
   T ab;
   if constexpr (sizeof(T) == 8) {
@@ -623,6 +649,10 @@ __device__ __forceinline__ T rlow_rhigh(T dens, T u, T beta, T r_low, T r_high, 
   using M = Math<T>;
   const T b2 = beta * beta;
   const T Tr = M::div(r_high * b2 + r_low, T(1) + b2);
+  if constexpr (sizeof(T) == 4) {  // c^2 cancels: m_p/m_e folded into one constant
+    const T te = T(kMP / kME) * u * (ge - T(1)) * (gi - T(1));
+    return M::div(te, dens * ((gi - T(1)) + (ge - T(1)) * Tr));
+  }
   T te = T(kCL * kCL) * (T(kMP) * u * (ge - T(1)) * (gi - T(1)));
   te = M::div(te, dens * ((gi - T(1)) + (ge - T(1)) * Tr));
   return te / T(kME * kCL * kCL);
