@@ -373,25 +373,47 @@ __global__ void __launch_bounds__(128 * kPairTiles) k_render_pair(SnapView sv, R
 // carries (state, dt) from the geodesic wave, ring B the eight interpolated
 // primitives and the state in the emission's precision (FP32 values are what
 // the FP32 emission would round them to) and dt in FP64 (0: nothing to add).
-constexpr int kTriDepth = 2;
-constexpr int kTriCompB = 15;  // P[8], s[1..7]
-constexpr size_t kTriRingA = (size_t)kPairTiles * kTriDepth * kPairComp * 64;  // doubles
-constexpr size_t kTriRingBd = (size_t)kPairTiles * kTriDepth * 64;              // doubles (dt)
-constexpr size_t kTriRingBt = (size_t)kPairTiles * kTriDepth * kTriCompB * 64;  // elements of T
+// Ring depths: ring A (geodesic -> gather) holds 3 rounds in mixed, 2 in
+// FP64 (LDS), ring B (gather -> emission) 2.
+template <bool MIXED>
+struct TriRings {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  static constexpr int kDepthA = MIXED ? 3 : 2, kDepthB = 2;
+  static constexpr int kCompB = 15;                                                  // P[8], s[1..7]
+  static constexpr size_t kA = (size_t)kPairTiles * kDepthA * kPairComp * 64;        // doubles
+  static constexpr size_t kBd = (size_t)kPairTiles * kDepthB * 64;                   // doubles (dt)
+  static constexpr size_t kBt = (size_t)kPairTiles * kDepthB * kCompB * 64;          // elements of T
+  static constexpr size_t kLdsExtra = (kA + kBd) * sizeof(double) + kBt * sizeof(T) + kPairTiles * 8 * sizeof(int);
+};
 
 template <bool MIXED>
 constexpr size_t tri_lds_extra() {
-  using T = typename std::conditional<MIXED, float, double>::type;
-  return (kTriRingA + kTriRingBd) * sizeof(double) + kTriRingBt * sizeof(T) + kPairTiles * 8 * sizeof(int);
+  return TriRings<MIXED>::kLdsExtra;
 }
 
-// Waits for round t of a ring: 1 ready, 0 the writer finished after fewer
-// rounds, -1 failure (the writer gave up, or kSpinLimit polls).
-__device__ __forceinline__ int wait_round(int* produced, int* done, int t) {
+// Polls *cnt until it reaches target: the value read, or -1 after kSpinLimit
+// polls.
+__device__ __forceinline__ int wait_value(int* cnt, int target) {
+  for (int spin = 0; spin < kSpinLimit; ++spin) {
+    const int v = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+    if (v >= target) return v;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return -1;
+}
+
+// Waits for round t of a ring: 1 ready (*avail = rounds published), 0 the
+// writer finished after fewer rounds, -1 failure (the writer gave up, or
+// kSpinLimit polls).
+__device__ __forceinline__ int wait_round(int* produced, int* done, int t, int* avail) {
   for (int spin = 0; spin < kSpinLimit; ++spin) {
     const int pr = __builtin_amdgcn_readfirstlane(
         __hip_atomic_load(produced, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
-    if (pr > t) return 1;
+    if (pr > t) {
+      *avail = pr;
+      return 1;
+    }
     const int dn = __builtin_amdgcn_readfirstlane(
         __hip_atomic_load(done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
     if (dn == -2) return -1;
@@ -406,12 +428,14 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
                                                                   int32_t* __restrict__ steps, int tab_doubles,
                                                                   int strided) {
   using T = typename std::conditional<MIXED, float, double>::type;
+  using R = TriRings<MIXED>;
+  constexpr int DA = R::kDepthA, DB = R::kDepthB, CB = R::kCompB;
   extern __shared__ double lds[];
   double* ringA_all = lds + tab_doubles;
-  double* ringBd_all = ringA_all + kTriRingA;
-  T* ringBt_all = reinterpret_cast<T*>(ringBd_all + kTriRingBd);
+  double* ringBd_all = ringA_all + R::kA;
+  T* ringBt_all = reinterpret_cast<T*>(ringBd_all + R::kBd);
   // per tile: producedA, consumedA, doneA, producedB, consumedB, doneB, pad, pad
-  int* ctr = reinterpret_cast<int*>(ringBt_all + kTriRingBt);
+  int* ctr = reinterpret_cast<int*>(ringBt_all + R::kBt);
   if (threadIdx.x < 8 * kPairTiles) {
     const int f = threadIdx.x & 7;
     ctr[threadIdx.x] = (f == 2 || f == 5) ? -1 : 0;
@@ -426,9 +450,9 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   // stages on three SIMDs instead of sharing one.
   const int stage = wave / kPairTiles;
   const int tl = (wave + (kPairTiles - 1) * stage) % kPairTiles;
-  double* ringA = ringA_all + (size_t)tl * kTriDepth * kPairComp * 64 + lane;
-  double* ringBd = ringBd_all + (size_t)tl * kTriDepth * 64 + lane;
-  T* ringBt = ringBt_all + (size_t)tl * kTriDepth * kTriCompB * 64 + lane;
+  double* ringA = ringA_all + (size_t)tl * DA * kPairComp * 64 + lane;
+  double* ringBd = ringBd_all + (size_t)tl * DB * 64 + lane;
+  T* ringBt = ringBt_all + (size_t)tl * DB * CB * 64 + lane;
   int* c = ctr + 8 * tl;
   const unsigned nwg = gridDim.x, wg = blockIdx.x, per = nwg / 8;
   const unsigned lwg = (nwg % 8 == 0) ? (wg % 8) * per + wg / 8 : wg;
@@ -458,81 +482,92 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
     }
     const bool zone = MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0);
     bool ok = true;
-    int t = 0;
+    int t = 0, freed = 0;  // freed: ring A rounds the gather wave has read (last value seen)
     while (__any(alive)) {
-      if (!wait_at_least(c + 1, t - kTriDepth + 1)) {
-        ok = false;
-        break;
+      if (t - DA + 1 > freed) {
+        freed = wait_value(c + 1, t - DA + 1);
+        if (freed < 0) {
+          ok = false;
+          break;
+        }
       }
-      double* out = ringA + (size_t)(t % kTriDepth) * kPairComp * 64;
+      double* out = ringA + (size_t)(t % DA) * kPairComp * 64;
+      if (alive && (k >= rp.N || dt == 0.0)) alive = false;
+      double nw[8];
+      if (alive) {
+        if constexpr (!MIXED) {
+          rk4_step<double>(s, dt, rp.a, nw);
+        } else {
+          const bool fp64 = zone && ((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
+          if (fp64) rk4_step<double>(s, dt, rp.a, nw);
+          else rk4_step_mixed(s, dt, (float)rp.a, nw);
+        }
+      }
+      // release the previous round here: its slot writes were issued a whole
+      // RK4 step ago, so the release fence does not wait on them
+      if (t > 0) publish(c + 0, t);
       bool wrote = false;
       if (alive) {
-        if (k >= rp.N || dt == 0.0) {
+        double rn, dtn;
+        if constexpr (!MIXED) {
+          rn = radius_cal(nw[1], nw[2], nw[3], rp.a);
+          dtn = step_rule(rn, rp.rEH, rp.div, rp.tol);
+        } else {
+          rn = radius_f32(nw[1], nw[2], nw[3], rp.a);
+          dtn = step_rule_fast(rn, rp.rEH, rp.inv_div, rp.div, rp.tol);
+        }
+        if (dtn == 0.0) {
           alive = false;
         } else {
-          double nw[8];
-          if constexpr (!MIXED) {
-            rk4_step<double>(s, dt, rp.a, nw);
-          } else {
-            const bool fp64 = zone && ((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
-            if (fp64) rk4_step<double>(s, dt, rp.a, nw);
-            else rk4_step_mixed(s, dt, (float)rp.a, nw);
-          }
-          double rn, dtn;
-          if constexpr (!MIXED) {
-            rn = radius_cal(nw[1], nw[2], nw[3], rp.a);
-            dtn = step_rule(rn, rp.rEH, rp.div, rp.tol);
-          } else {
-            rn = radius_f32(nw[1], nw[2], nw[3], rp.a);
-            dtn = step_rule_fast(rn, rp.rEH, rp.inv_div, rp.div, rp.tol);
-          }
-          if (dtn == 0.0) {
-            alive = false;
-          } else {
-            ++nacc;
-            if (k + 1 < rp.N) {  // row N is never emitted
+          ++nacc;
+          if (k + 1 < rp.N) {  // row N is never emitted
 #pragma unroll
-              for (int q = 0; q < 8; ++q) out[q * 64] = nw[q];
-              out[8 * 64] = dt;
-              wrote = true;
-            }
-            if (zone) drdl = (rn - r) / dt;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) s[q] = nw[q];
-            dt = dtn;
-            r = rn;
-            if (++k >= rp.N) alive = false;
+            for (int q = 0; q < 8; ++q) out[q * 64] = nw[q];
+            out[8 * 64] = dt;
+            wrote = true;
           }
+          if (zone) drdl = (rn - r) / dt;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) s[q] = nw[q];
+          dt = dtn;
+          r = rn;
+          if (++k >= rp.N) alive = false;
         }
       }
       if (!wrote) out[8 * 64] = 0.0;
-      publish(c + 0, ++t);
+      ++t;
     }
+    if (ok) publish(c + 0, t);
     publish(c + 2, ok ? t : -2);
     if (valid && steps) steps[p] = ok ? nacc : -1;
     return;
   }
   if (stage == 1) {  // gather: lookup, cell loads, interpolation
     bool ok = true;
-    int t = 0;
+    int t = 0, avail = 0, freed = 0;  // rounds of ring A published, of ring B read (last values seen)
     for (;; ++t) {
-      const int st = wait_round(c + 0, c + 2, t);
-      if (st <= 0) {
-        ok = st == 0;
-        break;
+      if (t >= avail) {
+        const int st = wait_round(c + 0, c + 2, t, &avail);
+        if (st <= 0) {
+          ok = st == 0;
+          break;
+        }
       }
-      if (!wait_at_least(c + 4, t - kTriDepth + 1)) {
-        ok = false;
-        break;
+      if (t - DB + 1 > freed) {
+        freed = wait_value(c + 4, t - DB + 1);
+        if (freed < 0) {
+          ok = false;
+          break;
+        }
       }
-      const double* in = ringA + (size_t)(t % kTriDepth) * kPairComp * 64;
+      const double* in = ringA + (size_t)(t % DA) * kPairComp * 64;
       double pdt = in[8 * 64];
       double ps[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) ps[q] = in[q * 64];
       publish(c + 1, t + 1);  // ring A slot is in registers
-      T* ob = ringBt + (size_t)(t % kTriDepth) * kTriCompB * 64;
-      double* obd = ringBd + (size_t)(t % kTriDepth) * 64;
+      T* ob = ringBt + (size_t)(t % DB) * CB * 64;
+      double* obd = ringBd + (size_t)(t % DB) * 64;
       T P[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) P[q] = T(0);
@@ -558,14 +593,17 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   // emission and transfer, in step order
   double I = 0.0, trans = 1.0;
   bool ok = true;
+  int avail = 0;  // rounds of ring B published (last value seen)
   for (int t = 0;; ++t) {
-    const int st = wait_round(c + 3, c + 5, t);
-    if (st <= 0) {
-      ok = st == 0;
-      break;
+    if (t >= avail) {
+      const int st = wait_round(c + 3, c + 5, t, &avail);
+      if (st <= 0) {
+        ok = st == 0;
+        break;
+      }
     }
-    const T* ib = ringBt + (size_t)(t % kTriDepth) * kTriCompB * 64;
-    const double pdt = ringBd[(size_t)(t % kTriDepth) * 64];
+    const T* ib = ringBt + (size_t)(t % DB) * CB * 64;
+    const double pdt = ringBd[(size_t)(t % DB) * 64];
     T P[8];
     double ps[8];
 #pragma unroll
