@@ -694,7 +694,10 @@ static int choose_form(const SnapView& sv, const RenderDev& rp, int precision, i
     set_error("mk_render: no device");
     return MK_ERR_HIP;
   }
-  if (rp.ntiles * 4 <= 7L * 4 * cus) *form = fits3 ? MK_RENDER_TRIPLE : MK_RENDER_PAIRED;
+  // triple form up to 4 tiles per SIMD (512^2 on 256 CUs); the paired form
+  // (when the triple form's rings do not fit) up to 1.75 tiles per SIMD
+  if (fits3 && rp.ntiles <= 16L * cus) *form = MK_RENDER_TRIPLE;
+  else if (rp.ntiles * 4 <= 7L * 4 * cus) *form = MK_RENDER_PAIRED;
   return MK_OK;
 }
 

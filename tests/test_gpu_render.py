@@ -195,10 +195,11 @@ def test_kernel_forms_agree_bit_for_bit(torus, precision, res):
 
 
 def test_form_choice(torus):
-    """Small launches (up to 1.75 tiles per SIMD) run the triple form, larger
-    ones the classic form; a forced form is honoured."""
+    """Launches of up to 4 tiles per SIMD (512^2 on 256 CUs) run the triple
+    form, larger ones the classic form; a forced form is honoured."""
     for prec in ("plain64", "mixed"):
         assert _form_of(torus, images.render_params(0.5, resolution=256, precision=prec)) == "triple"
+        assert _form_of(torus, images.render_params(0.5, resolution=512, precision=prec)) == "triple"
         assert _form_of(torus, images.render_params(0.5, resolution=1024, precision=prec)) == "classic"
         assert _form_of(torus, images.render_params(0.5, resolution=1024, precision=prec, form="paired")) == "paired"
 
