@@ -188,6 +188,11 @@ typedef struct mk_render_params {
 #define MK_RENDER_CLASSIC 1
 #define MK_RENDER_PAIRED 2
 #define MK_RENDER_TRIPLE 4
+/* Testing aid for the paired and triple forms: every bounded wait between
+ * the waves of a tile gives up after one poll, so the failure path runs (a
+ * wave that gives up marks its ring failed, and the tile's pixels are NaN
+ * and its step counts -1).  Normal waits give up after 2^24 polls. */
+#define MK_RENDER_WAIT_TEST 256
 
 /* make_image  (images.py:30-144) for a device-resident snapshot: camera,
  * geodesic, sampling, emission and transfer fused per ray; d_image

@@ -430,11 +430,11 @@ struct Stencil {
   double fr[3];
 };
 
-__device__ __forceinline__ Stencil locate(const SnapView& sv, const double* meta, const int* grid, double x,
-                                          double y, double z) {
+// Stencil from a known block record (mb >= 0, m its record) or none (mb < 0).
+__device__ __forceinline__ Stencil stencil_in(const SnapView& sv, int mb, const double m[kMeta], double x, double y,
+                                              double z) {
   Stencil st;
-  double m[kMeta];
-  st.mb = find_block_rec(sv, meta, grid, x, y, z, m);
+  st.mb = mb;
   st.base = 0;
   st.fr[0] = st.fr[1] = st.fr[2] = 0.0;
   if (st.mb < 0) return st;
@@ -452,6 +452,23 @@ __device__ __forceinline__ Stencil locate(const SnapView& sv, const double* meta
   const long n2 = sv.n2;
   st.base = 2 * (((long)st.mb * n2 + id[2]) * n2 * n2 + (long)id[1] * n2 + id[0]);
   return st;
+}
+
+__device__ __forceinline__ Stencil locate(const SnapView& sv, const double* meta, const int* grid, double x,
+                                          double y, double z) {
+  double m[kMeta];
+  const int mb = find_block_rec(sv, meta, grid, x, y, z, m);
+  return stencil_in(sv, mb, m, x, y, z);
+}
+
+// locate() with the caller's last block (cmb, its record crec; cmb < 0: none).
+// Leaf blocks do not overlap (left-open, right-closed boxes), so a point
+// inside the cached block lies in no other block and the table lookup is
+// skipped: the same stencil as locate().
+__device__ __forceinline__ Stencil locate_cached(const SnapView& sv, const double* meta, const int* grid, double x,
+                                                 double y, double z, int& cmb, double crec[kMeta]) {
+  if (!(cmb >= 0 && in_box(crec, x, y, z))) cmb = find_block_rec(sv, meta, grid, x, y, z, crec);
+  return stencil_in(sv, cmb, crec, x, y, z);
 }
 
 // The 8 corner records of a stencil: 4 rows of 64 contiguous bytes.

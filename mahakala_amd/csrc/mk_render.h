@@ -10,6 +10,7 @@ struct RenderDev {
   long ntiles;
   ImageParams ip;
   double fp64_r, ring_r, ring_drdl;
+  int spin_limit;  // polls before a bounded wait gives up (paired and triple forms)
 };
 
 int fill_render_dev(const mk_render_params* p, RenderDev* rp);

@@ -196,9 +196,9 @@ constexpr size_t kPairRingDoubles = (size_t)kPairTiles * kPairDepth * kPairComp 
 constexpr size_t kPairLdsExtra = kPairRingDoubles * sizeof(double) + kPairTiles * 4 * sizeof(int);
 constexpr int kSpinLimit = 1 << 24;
 
-// Polls *cnt (LDS) until it reaches target; false after kSpinLimit polls.
-__device__ __forceinline__ bool wait_at_least(int* cnt, int target) {
-  for (int spin = 0; spin < kSpinLimit; ++spin) {
+// Polls *cnt (LDS) until it reaches target; false after `limit` polls.
+__device__ __forceinline__ bool wait_at_least(int* cnt, int target, int limit) {
+  for (int spin = 0; spin < limit; ++spin) {
     const int v = __builtin_amdgcn_readfirstlane(
         __hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
     if (v >= target) return true;
@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(128 * kPairTiles) k_render_pair(SnapView sv, R
     bool ok = true;
     int t = 0;
     while (__any(alive)) {
-      if (!wait_at_least(consumed, t - kPairDepth + 1)) {
+      if (!wait_at_least(consumed, t - kPairDepth + 1, rp.spin_limit)) {
         ok = false;
         break;
       }
@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(128 * kPairTiles) k_render_pair(SnapView sv, R
   for (int t = 0;; ++t) {
     // round t is ready, or the geodesic wave has finished after fewer rounds
     int state = 0;
-    for (int spin = 0; spin < kSpinLimit; ++spin) {
+    for (int spin = 0; spin < rp.spin_limit; ++spin) {
       const int pr = __builtin_amdgcn_readfirstlane(
           __hip_atomic_load(produced, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
       if (pr > t) {
@@ -391,10 +391,10 @@ constexpr size_t tri_lds_extra() {
   return TriRings<MIXED>::kLdsExtra;
 }
 
-// Polls *cnt until it reaches target: the value read, or -1 after kSpinLimit
+// Polls *cnt until it reaches target: the value read, or -1 after `limit`
 // polls.
-__device__ __forceinline__ int wait_value(int* cnt, int target) {
-  for (int spin = 0; spin < kSpinLimit; ++spin) {
+__device__ __forceinline__ int wait_value(int* cnt, int target, int limit) {
+  for (int spin = 0; spin < limit; ++spin) {
     const int v = __builtin_amdgcn_readfirstlane(
         __hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
     if (v >= target) return v;
@@ -405,9 +405,9 @@ __device__ __forceinline__ int wait_value(int* cnt, int target) {
 
 // Waits for round t of a ring: 1 ready (*avail = rounds published), 0 the
 // writer finished after fewer rounds, -1 failure (the writer gave up, or
-// kSpinLimit polls).
-__device__ __forceinline__ int wait_round(int* produced, int* done, int t, int* avail) {
-  for (int spin = 0; spin < kSpinLimit; ++spin) {
+// `limit` polls).
+__device__ __forceinline__ int wait_round(int* produced, int* done, int t, int* avail, int limit) {
+  for (int spin = 0; spin < limit; ++spin) {
     const int pr = __builtin_amdgcn_readfirstlane(
         __hip_atomic_load(produced, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
     if (pr > t) {
@@ -485,7 +485,7 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
     int t = 0, freed = 0;  // freed: ring A rounds the gather wave has read (last value seen)
     while (__any(alive)) {
       if (t - DA + 1 > freed) {
-        freed = wait_value(c + 1, t - DA + 1);
+        freed = wait_value(c + 1, t - DA + 1, rp.spin_limit);
         if (freed < 0) {
           ok = false;
           break;
@@ -547,14 +547,14 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
     int t = 0, avail = 0, freed = 0;  // rounds of ring A published, of ring B read (last values seen)
     for (;; ++t) {
       if (t >= avail) {
-        const int st = wait_round(c + 0, c + 2, t, &avail);
+        const int st = wait_round(c + 0, c + 2, t, &avail, rp.spin_limit);
         if (st <= 0) {
           ok = st == 0;
           break;
         }
       }
       if (t - DB + 1 > freed) {
-        freed = wait_value(c + 4, t - DB + 1);
+        freed = wait_value(c + 4, t - DB + 1, rp.spin_limit);
         if (freed < 0) {
           ok = false;
           break;
@@ -596,7 +596,7 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   int avail = 0;  // rounds of ring B published (last value seen)
   for (int t = 0;; ++t) {
     if (t >= avail) {
-      const int st = wait_round(c + 3, c + 5, t, &avail);
+      const int st = wait_round(c + 3, c + 5, t, &avail, rp.spin_limit);
       if (st <= 0) {
         ok = st == 0;
         break;
@@ -662,6 +662,7 @@ int fill_render_dev(const mk_render_params* p, RenderDev* rp) {
   rp->fp64_r = p->fp64_radius;
   rp->ring_r = p->ring_radius;
   rp->ring_drdl = p->ring_drdl;
+  rp->spin_limit = (p->flags & MK_RENDER_WAIT_TEST) ? 1 : kSpinLimit;
   return MK_OK;
 }
 

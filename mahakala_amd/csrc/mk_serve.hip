@@ -103,6 +103,8 @@ __device__ __forceinline__ void pop_tile(const ServeArgs& A, int& rid, int& tile
 struct Lane {
   double s[8], dt, r, f10, f11, best, r_prev, r_at;
   int k, n, besti;
+  int cmb;              // block of the ray's last sample (-1: none), and its record:
+  double crec[kMeta];   // the snapshot tables are in global memory here, not in LDS
 };
 
 __device__ __forceinline__ void shadow_begin(const ReqDev& q, int item, Lane& L) {
@@ -144,6 +146,7 @@ __device__ __forceinline__ bool item_begin(const ReqDev& q, int item, Lane& L) {
     L.f11 = 1.0;  // transmission
     L.k = 0;
     L.n = 0;
+    L.cmb = -1;
     return false;
   }
   L.f10 = 0.5;   // inner
@@ -210,7 +213,9 @@ __device__ __forceinline__ bool image_pass(const ReqDev& q, bool go, Lane& L, Pe
       fin = true;
     } else {
       if (L.k + 1 < q.N) {  // row N is never emitted
-        p.st = locate(q.sv, q.sv.meta, q.sv.grid, nw[1], nw[2], nw[3]);
+        // FP64 has no registers to spare for the cached record (it would spill)
+        if constexpr (MIXED) p.st = locate_cached(q.sv, q.sv.meta, q.sv.grid, nw[1], nw[2], nw[3], L.cmb, L.crec);
+        else p.st = locate(q.sv, q.sv.meta, q.sv.grid, nw[1], nw[2], nw[3]);
         fetch(q.sv, p.st, p.c);
 #pragma unroll
         for (int c = 0; c < 8; ++c) p.s[c] = nw[c];
@@ -285,6 +290,7 @@ __global__ void __launch_bounds__(256, 2) k_serve(const ReqDev* __restrict__ req
   int rid = __builtin_amdgcn_readfirstlane(A.i[3 * L_ + l]);
   int tile = __builtin_amdgcn_readfirstlane(A.i[4 * L_ + l]);
   Lane L{};
+  L.cmb = -1;
   bool live = false;
   if (rid >= 0) {
     live = A.i[5 * L_ + l] != 0;

@@ -194,6 +194,29 @@ def test_kernel_forms_agree_bit_for_bit(torus, precision, res):
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("form", ["paired", "triple"])
+def test_bounded_wait_failure_path(torus, form):
+    """With the wait-test flag every wait between the waves of a tile gives up
+    after one poll: the launch still completes, a tile whose wave gave up has
+    NaN pixels (and step counts -1 when its geodesic wave gave up), every other
+    pixel equals the normal render, and the next normal render is unaffected."""
+    p = images.render_params(0.5, camera_inclination=30, resolution=64, precision="mixed", form=form)
+    ref, sref = images.render(torus.device_snapshot, p, return_steps=True)
+    p.flags |= 256  # MK_RENDER_WAIT_TEST
+    img, st = images.render(torus.device_snapshot, p, return_steps=True)
+    bad = np.isnan(img)
+    print(f"[render] {form} wait test: {int(bad.sum())} NaN pixels, {int((st == -1).sum())} failed rays")
+    assert bad.any()
+    assert np.all(bad[st == -1])
+    np.testing.assert_array_equal(img[~bad], ref[~bad])
+    ok = st != -1
+    np.testing.assert_array_equal(st[ok], sref[ok])
+    p.flags &= ~256
+    again, sagain = images.render(torus.device_snapshot, p, return_steps=True)
+    np.testing.assert_array_equal(again, ref)
+    np.testing.assert_array_equal(sagain, sref)
+
+
 def test_form_choice(torus):
     """Launches of up to 4 tiles per SIMD (512^2 on 256 CUs) run the triple
     form, larger ones the classic form; a forced form is honoured."""
