@@ -283,7 +283,8 @@ def main():
         L.check(L.lib().mk_render_form(snap.handle, par[p], C.byref(form)))
         m = "true" if par[p].precision == 1 else "false"
         if form.value == images.FORMS["triple"]:
-            return f"k_render_tri<{m}>"
+            # the workload's 512-block tables are staged in LDS
+            return f"k_render_tri<{m}, true>"
         return f"k_render_pair<{m}>" if form.value == images.FORMS["paired"] else f"k_render<{m}, true>"
 
     def launch(p, steps_out=False):

@@ -179,12 +179,11 @@ typedef struct mk_render_params {
 /* mk_render_params.flags: kernel form.  By default the library runs a launch
  * of up to 4 8x8 tiles per SIMD in the triple form -- a geodesic wave, a
  * gather wave (block lookup, cell loads, interpolation) and an emission wave
- * per tile -- and a larger one in the classic form (one wave per tile).  The
- * paired form runs a geodesic wave and a transfer wave per tile; it replaces
- * the triple form (up to 1.75 tiles per SIMD) where the triple form's LDS
- * rings do not fit.  The bits force one form (the triple form falls back to
- * the paired one where its rings do not fit); all give the same image, bit
- * for bit. */
+ * per tile, with the snapshot's block tables in LDS where they fit and in
+ * global memory otherwise -- and a larger one in the classic form (one wave
+ * per tile).  The paired form (a geodesic wave and a transfer wave per tile)
+ * runs only when forced and its tables fit in LDS (else the classic form).
+ * The bits force one form; all give the same image, bit for bit. */
 #define MK_RENDER_CLASSIC 1
 #define MK_RENDER_PAIRED 2
 #define MK_RENDER_TRIPLE 4

@@ -423,7 +423,10 @@ __device__ __forceinline__ int wait_round(int* produced, int* done, int t, int* 
   return -1;
 }
 
-template <bool MIXED>
+// LDS: the block tables are staged in LDS; otherwise (a snapshot whose tables
+// exceed the LDS budget) they are read from global memory, and the gather
+// wave keeps each ray's last block record in registers.
+template <bool MIXED, bool LDS>
 __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, RenderDev rp, double* __restrict__ img,
                                                                   int32_t* __restrict__ steps, int tab_doubles,
                                                                   int strided) {
@@ -442,7 +445,8 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   }
   const double* meta;
   const int* grid;
-  stage_tables<true>(sv, lds, meta, grid);  // ends with a workgroup barrier
+  stage_tables<LDS>(sv, lds, meta, grid);  // with LDS: ends with a workgroup barrier
+  if constexpr (!LDS) __syncthreads();    // the counters are set
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // Wave w runs on SIMD w % 4 (the hardware's placement).  The three waves of
   // tile k sit on SIMDs k, k + 1 and k + 2: every SIMD holds one wave of each
@@ -545,6 +549,8 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   if (stage == 1) {  // gather: lookup, cell loads, interpolation
     bool ok = true;
     int t = 0, avail = 0, freed = 0;  // rounds of ring A published, of ring B read (last values seen)
+    int cmb = -1;                     // global tables: block of the ray's last sample and its record
+    double crec[kMeta];
     for (;; ++t) {
       if (t >= avail) {
         const int st = wait_round(c + 0, c + 2, t, &avail, rp.spin_limit);
@@ -572,7 +578,9 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
 #pragma unroll
       for (int q = 0; q < 8; ++q) P[q] = T(0);
       if (pdt != 0.0) {
-        const Stencil stc = locate(sv, meta, grid, ps[1], ps[2], ps[3]);
+        Stencil stc;
+        if constexpr (LDS) stc = locate(sv, meta, grid, ps[1], ps[2], ps[3]);
+        else stc = locate_cached(sv, meta, grid, ps[1], ps[2], ps[3], cmb, crec);
         if (stc.mb >= 0) {
           Corners cr;
           fetch(sv, stc, cr);
@@ -674,19 +682,48 @@ using namespace mk;
 // has fewer than about 1.75 tiles per SIMD (measured on MI355X: paired is
 // 15-24 % faster up to 1600 tiles on 1024 SIMDs, and 8-24 % slower from 2304
 // tiles, where three classic workgroups per CU keep more waves in flight).
+// Tables in LDS when they fit kMaxTableLds and, with the form's rings, the
+// 160 KB of a CU; the triple form also runs with the tables in global memory.
+template <bool MIXED, bool LDS>
+static int launch_tri(const SnapView& sv, const RenderDev& rp, double* d_image, int32_t* d_steps, size_t tab,
+                      hipStream_t st) {
+  long nwg = (rp.ntiles + kPairTiles - 1) / kPairTiles;
+  nwg = (nwg + 7) / 8 * 8;
+  const size_t lds = tab + tri_lds_extra<MIXED>();
+  // strided tiles when every workgroup of the launch is resident at once
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_tri<MIXED, LDS>, 192 * kPairTiles, lds);
+  if (e != hipSuccess) {
+    set_error("mk_render: occupancy query failed");
+    return MK_ERR_HIP;
+  }
+  const int strided = nwg <= (long)per_cu * cus;
+  k_render_tri<MIXED, LDS><<<dim3((unsigned)nwg), dim3(192 * kPairTiles), lds, st>>>(
+      sv, rp, d_image, d_steps, (int)(tab / sizeof(double)), strided);
+  MK_LAUNCH_CHECK();
+  return MK_OK;
+}
+
+static bool tri_tables_in_lds(const SnapView& sv, int precision) {
+  const size_t tab = (table_bytes(sv) + 15) / 16 * 16;
+  return table_bytes(sv) <= kMaxTableLds &&
+         tab + (precision == 1 ? tri_lds_extra<true>() : tri_lds_extra<false>()) <= 160 * 1024;
+}
+
 static int choose_form(const SnapView& sv, const RenderDev& rp, int precision, int flags, int* form) {
   const size_t tab = (table_bytes(sv) + 15) / 16 * 16;
-  const bool fits = table_bytes(sv) <= kMaxTableLds && tab + kPairLdsExtra <= 160 * 1024;
-  const bool fits3 =
-      fits && tab + (precision == 1 ? tri_lds_extra<true>() : tri_lds_extra<false>()) <= 160 * 1024;
+  const bool fits2 = table_bytes(sv) <= kMaxTableLds && tab + kPairLdsExtra <= 160 * 1024;
   *form = MK_RENDER_CLASSIC;
-  if (!fits || (flags & MK_RENDER_CLASSIC)) return MK_OK;
+  if (flags & MK_RENDER_CLASSIC) return MK_OK;
   if (flags & MK_RENDER_TRIPLE) {
-    *form = fits3 ? MK_RENDER_TRIPLE : MK_RENDER_PAIRED;
+    *form = MK_RENDER_TRIPLE;
     return MK_OK;
   }
   if (flags & MK_RENDER_PAIRED) {
-    *form = MK_RENDER_PAIRED;
+    if (fits2) *form = MK_RENDER_PAIRED;
     return MK_OK;
   }
   int dev = 0, cus = 0;
@@ -695,10 +732,8 @@ static int choose_form(const SnapView& sv, const RenderDev& rp, int precision, i
     set_error("mk_render: no device");
     return MK_ERR_HIP;
   }
-  // triple form up to 4 tiles per SIMD (512^2 on 256 CUs); the paired form
-  // (when the triple form's rings do not fit) up to 1.75 tiles per SIMD
-  if (fits3 && rp.ntiles <= 16L * cus) *form = MK_RENDER_TRIPLE;
-  else if (rp.ntiles * 4 <= 7L * 4 * cus) *form = MK_RENDER_PAIRED;
+  // triple form up to 4 tiles per SIMD (512^2 on 256 CUs)
+  if (rp.ntiles <= 16L * cus) *form = MK_RENDER_TRIPLE;
   return MK_OK;
 }
 
@@ -732,32 +767,12 @@ extern "C" int mk_render(const mk_snapshot* snap, const mk_render_params* p, dou
   rc = choose_form(sv, rp, p->precision, p->flags, &form);
   if (rc) return rc;
   if (form == MK_RENDER_TRIPLE) {
-    long nwg3 = (rp.ntiles + kPairTiles - 1) / kPairTiles;
-    nwg3 = (nwg3 + 7) / 8 * 8;
-    const int tabd = (int)(tab / sizeof(double));
-    // strided tiles when every workgroup of the launch is resident at once
-    const size_t lds3 = tab + (p->precision == 0 ? tri_lds_extra<false>() : tri_lds_extra<true>());
-    int dev = 0, cus = 0, per_cu = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess)
-      e = p->precision == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_tri<false>,
-                                                                           192 * kPairTiles, lds3)
-                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_tri<true>,
-                                                                           192 * kPairTiles, lds3);
-    if (e != hipSuccess) {
-      set_error("mk_render: occupancy query failed");
-      return MK_ERR_HIP;
-    }
-    const int strided = nwg3 <= (long)per_cu * cus;
+    const bool in_lds = tri_tables_in_lds(sv, p->precision);
     if (p->precision == 0)
-      k_render_tri<false><<<dim3((unsigned)nwg3), dim3(192 * kPairTiles), lds3, st>>>(sv, rp, d_image, d_steps, tabd,
-                                                                             strided);
-    else
-      k_render_tri<true><<<dim3((unsigned)nwg3), dim3(192 * kPairTiles), lds3, st>>>(sv, rp, d_image, d_steps, tabd,
-                                                                            strided);
-    MK_LAUNCH_CHECK();
-    return MK_OK;
+      return in_lds ? launch_tri<false, true>(sv, rp, d_image, d_steps, tab, st)
+                    : launch_tri<false, false>(sv, rp, d_image, d_steps, 0, st);
+    return in_lds ? launch_tri<true, true>(sv, rp, d_image, d_steps, tab, st)
+                  : launch_tri<true, false>(sv, rp, d_image, d_steps, 0, st);
   }
   if (form == MK_RENDER_PAIRED) {
     long nwg2 = (rp.ntiles + kPairTiles - 1) / kPairTiles;

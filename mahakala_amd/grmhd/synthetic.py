@@ -173,14 +173,14 @@ def torus_fields(x, y, z, a, rho0=1.0, fluid_gamma=13. / 9, beta=10.0, r_high=40
 
 
 def torus_snapshot(bhspin, layout="uniform", seed=0, turbulence_sigma=0.0, rho0=None, fluid_gamma=13. / 9,
-                   grf_n=128, chunk=64):
+                   grf_n=128, chunk=64, nblocks=8, n=32):
     """AthenaK-shaped datasets of the analytic torus.
 
-    layout "uniform": 512 = 8^3 blocks of 32^3 over +-40 M (a 256^3 grid);
-    layout "octree": the 3-level 512-leaf tree over +-50 M.  rho0 defaults to
-    a seed-fixed value from numpy PCG64(seed).
+    layout "uniform": nblocks^3 blocks of n^3 cells over +-40 M (default 512
+    blocks of 32^3, a 256^3 grid); layout "octree": the 3-level 512-leaf tree
+    over +-50 M.  rho0 defaults to a seed-fixed value from numpy PCG64(seed).
     """
-    geo = uniform_layout() if layout == "uniform" else octree_layout()
+    geo = uniform_layout(nblocks=nblocks, n=n) if layout == "uniform" else octree_layout()
     extent = 40.0 if layout == "uniform" else 50.0
     if rho0 is None:
         rng = np.random.Generator(np.random.PCG64(seed))
@@ -208,9 +208,10 @@ def torus_snapshot(bhspin, layout="uniform", seed=0, turbulence_sigma=0.0, rho0=
     return ds
 
 
-def uniform_torus_snapshot(bhspin=0.5, seed=0):
-    """256^3 uniform torus as 512 blocks of 32^3 over +-40 M."""
-    return torus_snapshot(bhspin, layout="uniform", seed=seed)
+def uniform_torus_snapshot(bhspin=0.5, seed=0, nblocks=8, n=32):
+    """Uniform torus over +-40 M as nblocks^3 blocks of n^3 cells (default:
+    256^3 cells as 512 blocks of 32^3)."""
+    return torus_snapshot(bhspin, layout="uniform", seed=seed, nblocks=nblocks, n=n)
 
 
 def octree_turbulent_snapshot(bhspin=0.94, seed=0):

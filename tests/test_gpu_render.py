@@ -194,6 +194,33 @@ def test_kernel_forms_agree_bit_for_bit(torus, precision, res):
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.fixture(scope="module")
+def torus_many_blocks():
+    """The torus as 4096 blocks of 8^3 cells: block tables of 393 KB, more than
+    the LDS budget, so the kernels read them from global memory."""
+    return AthenakFluidModel.from_datasets(syn.uniform_torus_snapshot(0.5, seed=0, nblocks=16, n=8), 0.5,
+                                           fluid_gamma=13. / 9)
+
+
+@pytest.mark.parametrize("precision", ["plain64", "mixed"])
+def test_triple_form_with_tables_in_global_memory(torus_many_blocks, precision):
+    """Tables too large for LDS: the triple form reads them from global memory
+    and its gather wave keeps each ray's last block record; image and step
+    counts equal the classic form's bit for bit, and the form is the default
+    for a 128^2 image."""
+    out = {}
+    for form in ("classic", "triple"):
+        p = images.render_params(0.5, camera_inclination=30, resolution=128, precision=precision, form=form)
+        assert _form_of(torus_many_blocks, p) == form
+        out[form] = images.render(torus_many_blocks.device_snapshot, p, return_steps=True)
+    (a, sa), (b, sb) = out["classic"], out["triple"]
+    assert a.max() > 0 and np.all(np.isfinite(b))
+    np.testing.assert_array_equal(sa, sb)
+    np.testing.assert_array_equal(a, b)
+    p = images.render_params(0.5, camera_inclination=30, resolution=128, precision=precision)
+    assert _form_of(torus_many_blocks, p) == "triple"
+
+
 @pytest.mark.parametrize("form", ["paired", "triple"])
 def test_bounded_wait_failure_path(torus, form):
     """With the wait-test flag every wait between the waves of a tile gives up
