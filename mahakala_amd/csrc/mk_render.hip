@@ -29,7 +29,9 @@ namespace mk {
 // consumed after the next RK4 step has been computed, so the gather latency
 // overlaps the geodesic arithmetic.  Contributions are still added in step
 // order, so the result is the same as sampling in place.
-template <bool MIXED>
+// CACHE (tables in global memory): the ray keeps its last block record and
+// skips the lookup while its samples stay in that block.
+template <bool MIXED, bool CACHE>
 __device__ __forceinline__ void trace_ray(const SnapView& sv, const double* meta, const int* grid, const RenderDev& rp,
                                           double x_img, double y_img, double& I_out, int& steps_out) {
   using T = typename std::conditional<MIXED, float, double>::type;
@@ -44,6 +46,8 @@ __device__ __forceinline__ void trace_ray(const SnapView& sv, const double* meta
   Stencil pst;
   Corners pc;
   double ps[8], pdt = 0.0;
+  int cmb = -1;
+  double crec[kMeta];
   // Outside the domain (all primitives zero) and wherever the density is
   // zero, the reference's j and alpha come out 0 (or NaN, then 0), and the
   // step adds exactly nothing: those samples skip the emission arithmetic.
@@ -108,7 +112,8 @@ __device__ __forceinline__ void trace_ray(const SnapView& sv, const double* meta
     if (dtn == 0.0) break;
     ++nacc;
     if (k + 1 < rp.N) {  // row N is never emitted
-      pst = locate(sv, meta, grid, nw[1], nw[2], nw[3]);
+      if constexpr (CACHE) pst = locate_cached(sv, meta, grid, nw[1], nw[2], nw[3], cmb, crec);
+      else pst = locate(sv, meta, grid, nw[1], nw[2], nw[3]);
       fetch(sv, pst, pc);
 #pragma unroll
       for (int q = 0; q < 8; ++q) ps[q] = nw[q];
@@ -170,7 +175,7 @@ __global__ void __launch_bounds__(256) k_render(SnapView sv, RenderDev rp, doubl
   if (!pixel_of_lane(rp, i, j)) return;
   double I;
   int n;
-  trace_ray<MIXED>(sv, meta, grid, rp, grid_coord(rp.lo, rp.step, i), grid_coord(rp.lo, rp.step, j), I, n);
+  trace_ray<MIXED, !LDS>(sv, meta, grid, rp, grid_coord(rp.lo, rp.step, i), grid_coord(rp.lo, rp.step, j), I, n);
   const long p = (long)i * rp.res + j;
   img[p] = I;
   if (steps) steps[p] = n;
