@@ -171,7 +171,7 @@ def _form_of(model, params):
 
 
 @pytest.mark.parametrize("precision", ["plain64", "mixed", "mixed-zone"])
-@pytest.mark.parametrize("res", [24, 256])
+@pytest.mark.parametrize("res", [24, 36, 256])
 def test_kernel_forms_agree_bit_for_bit(torus, precision, res):
     """The classic form (one wave per tile), the paired form (geodesic and
     transfer waves, samples handed over through LDS) and the triple form
@@ -204,19 +204,21 @@ def torus_many_blocks():
 
 @pytest.mark.parametrize("precision", ["plain64", "mixed"])
 def test_triple_form_with_tables_in_global_memory(torus_many_blocks, precision):
+    # 36^2: edge tiles with idle lanes; 128^2: full tiles
     """Tables too large for LDS: the triple form reads them from global memory
     and its gather wave keeps each ray's last block record; image and step
     counts equal the classic form's bit for bit, and the form is the default
     for a 128^2 image."""
-    out = {}
-    for form in ("classic", "triple"):
-        p = images.render_params(0.5, camera_inclination=30, resolution=128, precision=precision, form=form)
-        assert _form_of(torus_many_blocks, p) == form
-        out[form] = images.render(torus_many_blocks.device_snapshot, p, return_steps=True)
-    (a, sa), (b, sb) = out["classic"], out["triple"]
-    assert a.max() > 0 and np.all(np.isfinite(b))
-    np.testing.assert_array_equal(sa, sb)
-    np.testing.assert_array_equal(a, b)
+    for res in (36, 128):
+        out = {}
+        for form in ("classic", "triple"):
+            p = images.render_params(0.5, camera_inclination=30, resolution=res, precision=precision, form=form)
+            assert _form_of(torus_many_blocks, p) == form
+            out[form] = images.render(torus_many_blocks.device_snapshot, p, return_steps=True)
+        (a, sa), (b, sb) = out["classic"], out["triple"]
+        assert a.max() > 0 and np.all(np.isfinite(b))
+        np.testing.assert_array_equal(sa, sb)
+        np.testing.assert_array_equal(a, b)
     p = images.render_params(0.5, camera_inclination=30, resolution=128, precision=precision)
     assert _form_of(torus_many_blocks, p) == "triple"
 
