@@ -238,6 +238,71 @@ __global__ void __launch_bounds__(256) k_bisect(double a, double ci, double si, 
   radii[p] = inner;
 }
 
+// The same bisection, K halvings per round: each angle's group of 2^K lanes
+// integrates all 2^K - 1 midpoints the next K halvings could visit (the
+// nodes of a binary tree in heap order, each computed along its path with
+// the reference's (outer - inner) / 2 + inner), then one lane replays the K
+// halvings from the classifications.  The halvings, midpoints and result are
+// the sequential bisection's, in ceil(iters / K) integration rounds instead
+// of iters.
+__global__ void __launch_bounds__(1024) k_bisect_tree(double a, double ci, double si, double d, const double* cang,
+                                                      const double* sang, long n, int N, int iters, double rEH,
+                                                      int K, double* radii) {
+  __shared__ double br[2 * 1024];  // inner, outer per angle of the block
+  __shared__ int cls[1024];
+  const int gsz = 1 << K, G = blockDim.x >> K;
+  const int g = threadIdx.x >> K, j = threadIdx.x & (gsz - 1);
+  const long p = (long)blockIdx.x * G + g;
+  if (j == 0) {
+    br[2 * g] = 0.5;
+    br[2 * g + 1] = 10.0;
+  }
+  __syncthreads();
+  for (int done = 0; done < iters;) {
+    const int kr = min(K, iters - done);
+    int c = 0;
+    if (p < n && j < (1 << kr) - 1) {
+      // node j (heap order): the bits of j + 1 below its leading one are the
+      // path from the root, 1 = the ray fell in (inner = mid)
+      const int m = j + 1, depth = 31 - __builtin_clz(m);
+      double lo = br[2 * g], hi = br[2 * g + 1];
+      double mid = (hi - lo) / 2 + lo;
+      for (int b = depth - 1; b >= 0; --b) {
+        if ((m >> b) & 1) lo = mid;
+        else hi = mid;
+        mid = (hi - lo) / 2 + lo;
+      }
+      double sx[4], sv[4], st[8];
+      camera_ray(cang[p] * mid, sang[p] * mid, ci, si, d, sx, sv);
+      nullify(sx, sv, a, st);
+      WalkOut o;
+      walk_ray(st, N, 40.0, 1e-2, a, rEH, o);
+      c = o.sel_r < 100.0 ? 1 : 0;
+    }
+    cls[threadIdx.x] = c;
+    __syncthreads();
+    if (j == 0 && p < n) {
+      double lo = br[2 * g], hi = br[2 * g + 1];
+      int node = 0;
+      for (int l = 0; l < kr; ++l) {
+        const double mid = (hi - lo) / 2 + lo;
+        if (cls[(g << K) + node]) {
+          lo = mid;
+          node = 2 * node + 2;
+        } else {
+          hi = mid;
+          node = 2 * node + 1;
+        }
+      }
+      br[2 * g] = lo;
+      br[2 * g + 1] = hi;
+    }
+    __syncthreads();
+    done += kr;
+  }
+  if (j == 0 && p < n) radii[p] = br[2 * g];
+}
+
 __global__ void k_synchrotron(const double* Ne, const double* Th, const double* B, const double* pitch,
                               const double* nu, long n, int invariant, double rescale, double* j, double* al) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -583,7 +648,21 @@ int mk_shadow_bisect(double a, double inc, const double* h_angles, long n, int m
   if (e == hipSuccess) e = hipMemcpy(dc, hc.data(), n * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(ds, hs.data(), n * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    k_bisect<<<grid_for(n, 64), 64>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, horizon(a), dr);
+    // K halvings per round with 2^K lanes per angle: the fewest rounds with at
+    // most about 65 536 lanes at once, then the halvings spread evenly over
+    // those rounds (fewer lanes for the same number of rounds; the slowest
+    // integration of a round sets its time).  K = 1: one lane per angle.
+    int Kmax = 1;
+    while (Kmax < 10 && Kmax < iters && n * (2L << Kmax) <= 65536) ++Kmax;
+    const int rounds = (iters + Kmax - 1) / Kmax;
+    const int K = rounds > 0 ? (iters + rounds - 1) / rounds : 1;
+    if (K == 1 || iters <= 1) {
+      k_bisect<<<grid_for(n, 64), 64>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, horizon(a), dr);
+    } else {
+      const int block = (1 << K) >= 256 ? (1 << K) : 256, G = block >> K;
+      k_bisect_tree<<<(unsigned)((n + G - 1) / G), block>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, horizon(a),
+                                                             K, dr);
+    }
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(h_radii, dr, n * sizeof(double), hipMemcpyDeviceToHost);

@@ -115,6 +115,19 @@ def test_shadow_64_angles_and_select():
     np.testing.assert_allclose(sel, f["select_r"], rtol=1e-8)
 
 
+def test_shadow_rounds_of_several_halvings_equal_the_sequential_bisection():
+    """mk_shadow_bisect runs K halvings per round (2^K - 1 midpoints per angle
+    integrated at once, K from the angle count: 7 for 64 angles, 5 for 1024,
+    4 for 4096) and one lane per angle from 32768 angles on; the radii are the
+    same bits either way."""
+    ang = np.arange(4096) / 4096 * 2 * np.pi
+    seq = ma.find_shadow_bisection_angles(0.9, 60, np.tile(ang, 16))  # 65536 angles: one lane each
+    np.testing.assert_array_equal(seq[:4096], seq[4096:8192])
+    for sub in (ang[::64], ang[::4], ang):  # 64, 1024, 4096 angles
+        got = ma.find_shadow_bisection_angles(0.9, 60, sub)
+        np.testing.assert_array_equal(got, seq[:4096][np.isin(ang, sub)])
+
+
 def test_schwarzschild_shadow():
     r = ma.find_shadow_bisection_angles(0.0, 17.0, np.linspace(0, 6, 12))
     assert np.all(np.abs(r - np.sqrt(27)) < 9.5 / 2 ** 14 + 1e-3)
