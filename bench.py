@@ -238,6 +238,7 @@ def main():
     ap.add_argument("--no-heavy", action="store_true", help="skip the heavy queued stream (8 snapshots)")
     ap.add_argument("--no-target", action="store_true", help="skip the a=0.9 precision-target image")
     ap.add_argument("--no-sizes", action="store_true", help="skip the 512^2 and 1024^2 images")
+    ap.add_argument("--no-shadow", action="store_true", help="skip the shadow bisection timings")
     ap.add_argument("--form", choices=["auto", "classic", "paired", "triple"], default="auto",
                     help="render kernel form (auto: the library's choice)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -391,6 +392,25 @@ def main():
             del img2
         out["larger_images"] = dict(sizes, what="same scene and precision as value, one launch per image")
         print(f"[bench] larger images: {sizes}", file=sys.stderr, flush=True)
+
+    if not args.no_shadow:
+        # find_shadow_bisection(0.9, 60, 64) and a 4096-angle job: 14 halvings,
+        # each a full integration, reported as ray-integrations/s
+        from mahakala_amd.geodesics import find_shadow_bisection_angles
+        sh = {}
+        for n_ang in (64, 4096):
+            ang = np.arange(n_ang) / n_ang * 2 * np.pi
+            find_shadow_bisection_angles(0.9, 60, ang)
+            ts = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                find_shadow_bisection_angles(0.9, 60, ang)
+                ts.append(time.perf_counter() - t0)
+            t = float(np.median(ts))
+            sh[f"{n_ang}_angles"] = {"ms": round(t * 1e3, 3), "ray_integrations_per_s": round(n_ang * 14 / t, 1)}
+        out["shadow_bisection"] = dict(sh, what="a=0.9, i=60 deg, N=2000, error 1e-3 (14 halvings), host arrays in "
+                                                "and out (the reference call's boundary)")
+        print(f"[bench] shadow bisection: {sh}", file=sys.stderr, flush=True)
 
     if not args.no_target:
         # the precision target's case: a = 0.9, 256^2, i = 60 deg (make_image's
