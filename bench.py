@@ -284,8 +284,9 @@ def main():
         L.check(L.lib().mk_render_form(snap.handle, par[p], C.byref(form)))
         m = "true" if par[p].precision == 1 else "false"
         if form.value == images.FORMS["triple"]:
-            # the workload's 512-block tables are staged in LDS
-            return f"k_render_tri<{m}, true>"
+            # the workload's 512-block tables are staged in LDS; ring B holds
+            # 3 rounds in mixed, 2 in FP64
+            return f"k_render_tri<{m}, true, {3 if m == 'true' else 2}>"
         return f"k_render_pair<{m}>" if form.value == images.FORMS["paired"] else f"k_render<{m}, true>"
 
     def launch(p, steps_out=False):

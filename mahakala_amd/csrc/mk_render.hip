@@ -379,11 +379,12 @@ __global__ void __launch_bounds__(128 * kPairTiles) k_render_pair(SnapView sv, R
 // primitives and the state in the emission's precision (FP32 values are what
 // the FP32 emission would round them to) and dt in FP64 (0: nothing to add).
 // Ring depths: ring A (geodesic -> gather) holds 3 rounds in mixed, 2 in
-// FP64 (LDS), ring B (gather -> emission) 2.
-template <bool MIXED>
+// FP64 (LDS), ring B (gather -> emission) DB_ rounds (3 in mixed where LDS
+// allows, else 2).
+template <bool MIXED, int DB_>
 struct TriRings {
   using T = typename std::conditional<MIXED, float, double>::type;
-  static constexpr int kDepthA = MIXED ? 3 : 2, kDepthB = 2;
+  static constexpr int kDepthA = MIXED ? 3 : 2, kDepthB = DB_;
   static constexpr int kCompB = 15;                                                  // P[8], s[1..7]
   static constexpr size_t kA = (size_t)kPairTiles * kDepthA * kPairComp * 64;        // doubles
   static constexpr size_t kBd = (size_t)kPairTiles * kDepthB * 64;                   // doubles (dt)
@@ -391,9 +392,9 @@ struct TriRings {
   static constexpr size_t kLdsExtra = (kA + kBd) * sizeof(double) + kBt * sizeof(T) + kPairTiles * 8 * sizeof(int);
 };
 
-template <bool MIXED>
+template <bool MIXED, int DB_>
 constexpr size_t tri_lds_extra() {
-  return TriRings<MIXED>::kLdsExtra;
+  return TriRings<MIXED, DB_>::kLdsExtra;
 }
 
 // Polls *cnt until it reaches target: the value read, or -1 after `limit`
@@ -431,12 +432,12 @@ __device__ __forceinline__ int wait_round(int* produced, int* done, int t, int* 
 // LDS: the block tables are staged in LDS; otherwise (a snapshot whose tables
 // exceed the LDS budget) they are read from global memory, and the gather
 // wave keeps each ray's last block record in registers.
-template <bool MIXED, bool LDS>
+template <bool MIXED, bool LDS, int DB_>
 __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, RenderDev rp, double* __restrict__ img,
                                                                   int32_t* __restrict__ steps, int tab_doubles,
                                                                   int strided) {
   using T = typename std::conditional<MIXED, float, double>::type;
-  using R = TriRings<MIXED>;
+  using R = TriRings<MIXED, DB_>;
   constexpr int DA = R::kDepthA, DB = R::kDepthB, CB = R::kCompB;
   extern __shared__ double lds[];
   double* ringA_all = lds + tab_doubles;
@@ -689,33 +690,41 @@ using namespace mk;
 // tiles, where three classic workgroups per CU keep more waves in flight).
 // Tables in LDS when they fit kMaxTableLds and, with the form's rings, the
 // 160 KB of a CU; the triple form also runs with the tables in global memory.
-template <bool MIXED, bool LDS>
+template <bool MIXED, bool LDS, int DB_>
 static int launch_tri(const SnapView& sv, const RenderDev& rp, double* d_image, int32_t* d_steps, size_t tab,
                       hipStream_t st) {
   long nwg = (rp.ntiles + kPairTiles - 1) / kPairTiles;
   nwg = (nwg + 7) / 8 * 8;
-  const size_t lds = tab + tri_lds_extra<MIXED>();
+  const size_t lds = tab + tri_lds_extra<MIXED, DB_>();
   // strided tiles when every workgroup of the launch is resident at once
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_tri<MIXED, LDS>, 192 * kPairTiles, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_tri<MIXED, LDS, DB_>, 192 * kPairTiles, lds);
   if (e != hipSuccess) {
     set_error("mk_render: occupancy query failed");
     return MK_ERR_HIP;
   }
   const int strided = nwg <= (long)per_cu * cus;
-  k_render_tri<MIXED, LDS><<<dim3((unsigned)nwg), dim3(192 * kPairTiles), lds, st>>>(
+  k_render_tri<MIXED, LDS, DB_><<<dim3((unsigned)nwg), dim3(192 * kPairTiles), lds, st>>>(
       sv, rp, d_image, d_steps, (int)(tab / sizeof(double)), strided);
   MK_LAUNCH_CHECK();
   return MK_OK;
 }
 
-static bool tri_tables_in_lds(const SnapView& sv, int precision) {
+// The triple form's variant: 0 tables in LDS with 3 ring-B slots (mixed),
+// 1 tables in LDS with 2 slots, 2 tables in global memory (3 slots in mixed,
+// 2 in FP64).
+static int tri_variant(const SnapView& sv, int precision) {
   const size_t tab = (table_bytes(sv) + 15) / 16 * 16;
-  return table_bytes(sv) <= kMaxTableLds &&
-         tab + (precision == 1 ? tri_lds_extra<true>() : tri_lds_extra<false>()) <= 160 * 1024;
+  const bool small = table_bytes(sv) <= kMaxTableLds;
+  if (precision == 1) {
+    if (small && tab + tri_lds_extra<true, 3>() <= 160 * 1024) return 0;
+    if (small && tab + tri_lds_extra<true, 2>() <= 160 * 1024) return 1;
+    return 2;
+  }
+  return small && tab + tri_lds_extra<false, 2>() <= 160 * 1024 ? 1 : 2;
 }
 
 static int choose_form(const SnapView& sv, const RenderDev& rp, int precision, int flags, int* form) {
@@ -772,12 +781,13 @@ extern "C" int mk_render(const mk_snapshot* snap, const mk_render_params* p, dou
   rc = choose_form(sv, rp, p->precision, p->flags, &form);
   if (rc) return rc;
   if (form == MK_RENDER_TRIPLE) {
-    const bool in_lds = tri_tables_in_lds(sv, p->precision);
+    const int v = tri_variant(sv, p->precision);
     if (p->precision == 0)
-      return in_lds ? launch_tri<false, true>(sv, rp, d_image, d_steps, tab, st)
-                    : launch_tri<false, false>(sv, rp, d_image, d_steps, 0, st);
-    return in_lds ? launch_tri<true, true>(sv, rp, d_image, d_steps, tab, st)
-                  : launch_tri<true, false>(sv, rp, d_image, d_steps, 0, st);
+      return v == 1 ? launch_tri<false, true, 2>(sv, rp, d_image, d_steps, tab, st)
+                    : launch_tri<false, false, 2>(sv, rp, d_image, d_steps, 0, st);
+    if (v == 0) return launch_tri<true, true, 3>(sv, rp, d_image, d_steps, tab, st);
+    if (v == 1) return launch_tri<true, true, 2>(sv, rp, d_image, d_steps, tab, st);
+    return launch_tri<true, false, 3>(sv, rp, d_image, d_steps, 0, st);
   }
   if (form == MK_RENDER_PAIRED) {
     long nwg2 = (rp.ntiles + kPairTiles - 1) / kPairTiles;
