@@ -30,8 +30,12 @@ namespace mk {
 // overlaps the geodesic arithmetic.  Contributions are still added in step
 // order, so the result is the same as sampling in place.
 // CACHE (tables in global memory): the ray keeps its last block record and
-// skips the lookup while its samples stay in that block.
-template <bool MIXED, bool CACHE>
+// skips the lookup while its samples stay in that block.  IN_PLACE (mixed
+// precision with the tables in LDS): no pipeline; the sample is consumed
+// right after its step, so no corner records are held across the next RK4
+// step and the kernel fits three waves per SIMD (168 VGPRs), which hides the
+// load latency better than the pipeline does at two.
+template <bool MIXED, bool CACHE, bool IN_PLACE>
 __device__ __forceinline__ void trace_ray(const SnapView& sv, const double* meta, const int* grid, const RenderDev& rp,
                                           double x_img, double y_img, double& I_out, int& steps_out) {
   using T = typename std::conditional<MIXED, float, double>::type;
@@ -119,6 +123,10 @@ __device__ __forceinline__ void trace_ray(const SnapView& sv, const double* meta
       for (int q = 0; q < 8; ++q) ps[q] = nw[q];
       pdt = dt;
       pend = true;
+      if constexpr (IN_PLACE) {
+        consume();
+        pend = false;
+      }
     }
     if (zone) drdl = (rn - r) / dt;
 #pragma unroll
@@ -165,7 +173,7 @@ __device__ __forceinline__ void stage_tables(const SnapView& sv, double* lds, co
 }
 
 template <bool MIXED, bool LDS>
-__global__ void __launch_bounds__(256) k_render(SnapView sv, RenderDev rp, double* __restrict__ img,
+__global__ void __launch_bounds__(256, MIXED && LDS ? 3 : 2) k_render(SnapView sv, RenderDev rp, double* __restrict__ img,
                                                 int32_t* __restrict__ steps) {
   extern __shared__ double lds[];
   const double* meta;
@@ -175,7 +183,8 @@ __global__ void __launch_bounds__(256) k_render(SnapView sv, RenderDev rp, doubl
   if (!pixel_of_lane(rp, i, j)) return;
   double I;
   int n;
-  trace_ray<MIXED, !LDS>(sv, meta, grid, rp, grid_coord(rp.lo, rp.step, i), grid_coord(rp.lo, rp.step, j), I, n);
+  trace_ray<MIXED, !LDS, MIXED && LDS>(sv, meta, grid, rp, grid_coord(rp.lo, rp.step, i), grid_coord(rp.lo, rp.step, j),
+                                       I, n);
   const long p = (long)i * rp.res + j;
   img[p] = I;
   if (steps) steps[p] = n;
