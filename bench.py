@@ -287,7 +287,10 @@ def main():
             # the workload's 512-block tables are staged in LDS; ring B holds
             # 3 rounds in mixed, 2 in FP64
             return f"k_render_tri<{m}, true, {3 if m == 'true' else 2}>"
-        return f"k_render_pair<{m}>" if form.value == images.FORMS["paired"] else f"k_render<{m}, true>"
+        if form.value == images.FORMS["paired"]:
+            return f"k_render_pair<{m}>"
+        # classic: tables in LDS; mixed runs three workgroups per CU with in-place samples
+        return f"k_render<{m}, true, {m}>"
 
     def launch(p, steps_out=False):
         def fn():

@@ -31,7 +31,7 @@ namespace mk {
 // order, so the result is the same as sampling in place.
 // CACHE (tables in global memory): the ray keeps its last block record and
 // skips the lookup while its samples stay in that block.  IN_PLACE (mixed
-// precision with the tables in LDS): no pipeline; the sample is consumed
+// precision, tables in LDS, three workgroups per CU): no pipeline; the sample is consumed
 // right after its step, so no corner records are held across the next RK4
 // step and the kernel fits three waves per SIMD (168 VGPRs), which hides the
 // load latency better than the pipeline does at two.
@@ -172,9 +172,11 @@ __device__ __forceinline__ void stage_tables(const SnapView& sv, double* lds, co
   }
 }
 
-template <bool MIXED, bool LDS>
-__global__ void __launch_bounds__(256, MIXED && LDS ? 3 : 2) k_render(SnapView sv, RenderDev rp, double* __restrict__ img,
-                                                int32_t* __restrict__ steps) {
+// IN_PLACE (mixed, tables in LDS small enough for three workgroups per CU):
+// samples consumed right after their step, three waves per SIMD.
+template <bool MIXED, bool LDS, bool IN_PLACE>
+__global__ void __launch_bounds__(256, IN_PLACE ? 3 : 2) k_render(SnapView sv, RenderDev rp, double* __restrict__ img,
+                                                                  int32_t* __restrict__ steps) {
   extern __shared__ double lds[];
   const double* meta;
   const int* grid;
@@ -183,8 +185,8 @@ __global__ void __launch_bounds__(256, MIXED && LDS ? 3 : 2) k_render(SnapView s
   if (!pixel_of_lane(rp, i, j)) return;
   double I;
   int n;
-  trace_ray<MIXED, !LDS, MIXED && LDS>(sv, meta, grid, rp, grid_coord(rp.lo, rp.step, i), grid_coord(rp.lo, rp.step, j),
-                                       I, n);
+  trace_ray<MIXED, !LDS, IN_PLACE>(sv, meta, grid, rp, grid_coord(rp.lo, rp.step, i), grid_coord(rp.lo, rp.step, j),
+                                   I, n);
   const long p = (long)i * rp.res + j;
   img[p] = I;
   if (steps) steps[p] = n;
@@ -811,11 +813,13 @@ extern "C" int mk_render(const mk_snapshot* snap, const mk_render_params* p, dou
     return MK_OK;
   }
   if (p->precision == 0) {
-    if (use_lds) k_render<false, true><<<grid, block, lds, st>>>(sv, rp, d_image, d_steps);
-    else k_render<false, false><<<grid, block, 0, st>>>(sv, rp, d_image, d_steps);
+    if (use_lds) k_render<false, true, false><<<grid, block, lds, st>>>(sv, rp, d_image, d_steps);
+    else k_render<false, false, false><<<grid, block, 0, st>>>(sv, rp, d_image, d_steps);
   } else {
-    if (use_lds) k_render<true, true><<<grid, block, lds, st>>>(sv, rp, d_image, d_steps);
-    else k_render<true, false><<<grid, block, 0, st>>>(sv, rp, d_image, d_steps);
+    // three workgroups per CU when their tables fit the CU's 160 KB of LDS
+    if (use_lds && 3 * lds <= 160 * 1024) k_render<true, true, true><<<grid, block, lds, st>>>(sv, rp, d_image, d_steps);
+    else if (use_lds) k_render<true, true, false><<<grid, block, lds, st>>>(sv, rp, d_image, d_steps);
+    else k_render<true, false, false><<<grid, block, 0, st>>>(sv, rp, d_image, d_steps);
   }
   MK_LAUNCH_CHECK();
   return MK_OK;
