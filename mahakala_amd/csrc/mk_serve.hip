@@ -47,6 +47,7 @@ struct ReqDev {
   double* out;
   const double* cang;  // shadow: cos / sin of the angles
   const double* sang;
+  const int* order;    // image (tiled): the tile taken k-th, longest expected first
 };
 
 // A segment of the work ring: the tiles of one request (64 work items each:
@@ -125,7 +126,7 @@ __device__ __forceinline__ void shadow_begin(const ReqDev& q, int item, Lane& L)
 // resolution allows, so a wave's 64 items are neighbouring rays.
 __device__ __forceinline__ int pixel_of_item(const ReqDev& q, int item) {
   if (!q.tiled) return item;
-  const int tile = item >> 6, w = item & 63, tx = q.res >> 3;
+  const int tile = q.order[item >> 6], w = item & 63, tx = q.res >> 3;
   const int i = (tile / tx) * 8 + (w >> 3), j = (tile % tx) * 8 + (w & 7);
   return i * q.res + j;
 }
@@ -429,7 +430,9 @@ struct mk_server {
     double* h_out;
     Clock::time_point t0;
     std::vector<double> cs;  // shadow: cos, sin of the angles (copy source)
+    std::vector<int> ord;    // image: tile order (copy source)
   };
+  std::map<std::pair<int, double>, std::vector<int>> orders;  // tile orders by (resolution, fov)
   std::deque<Pending> waiting;
   std::map<int, Flight> flight;
   std::vector<int> free_rids;
@@ -493,6 +496,37 @@ void arena_free(mk_server* s, size_t off, size_t bytes) {
     }                                                                      \
   } while (0)
 
+size_t image_out_bytes(int res) { return align_up((size_t)res * res * sizeof(double)); }
+
+size_t image_bytes(int res) {
+  const size_t tiles = (res % 8 == 0) ? (size_t)(res / 8) * (res / 8) : 0;
+  return image_out_bytes(res) + align_up(tiles * sizeof(int));
+}
+
+// The order in which an image's 8x8 tiles are handed out: by the distance of
+// the tile centre from the ring of radius 5.2 M in the image plane (about
+// where the photon ring of a distant camera falls, whose rays take the most
+// steps), nearest first, so that a request's longest tiles start first and
+// its last tile ends sooner.  Only the schedule changes: every ray's
+// arithmetic is the same in any order.
+std::vector<int> tile_order(mk_server* s, int res, double fov, double lo, double step) {
+  const auto key = std::make_pair(res, fov);
+  auto it = s->orders.find(key);
+  if (it != s->orders.end()) return it->second;
+  const int tx = res / 8, nt = tx * tx;
+  std::vector<double> d(nt);
+  for (int t = 0; t < nt; ++t) {
+    const double x = (16.0 * (t / tx) + 8.0) * step + lo, y = (16.0 * (t % tx) + 8.0) * step + lo;
+    d[t] = std::fabs(std::sqrt(x * x + y * y) - 5.2);
+  }
+  std::vector<int> ord(nt);
+  for (int t = 0; t < nt; ++t) ord[t] = t;
+  std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) { return d[u] < d[v]; });
+  if (s->orders.size() >= 256) s->orders.clear();
+  s->orders.emplace(key, ord);
+  return ord;
+}
+
 // Moves waiting requests into flight while they fit (caller holds mu).
 bool admit(mk_server* s) {
   while (!s->waiting.empty() && !s->free_rids.empty() && s->tail - s->head < s->seg_cap) {
@@ -535,6 +569,10 @@ bool admit(mk_server* s) {
       q.ip = rd.ip;
       q.sv = s->snaps[r.snapshot]->view();
       q.out = (double*)(s->d_arena + off);
+      if (q.tiled) {
+        f.ord = tile_order(s, r.resolution, r.fov, rd.lo, rd.step);
+        q.order = (const int*)(s->d_arena + off + image_out_bytes(r.resolution));
+      }
     } else {
       const mk_shadow_request& r = p.sh;
       const int n = r.n_angles;
@@ -575,6 +613,10 @@ bool admit(mk_server* s) {
     SRV_HIP(hipMemcpyAsync(s->d_segs + (s->tail % s->seg_cap), &sg, sizeof sg, hipMemcpyHostToDevice, s->stream));
     ++s->tail;
     s->flight[rid] = std::move(f);
+    if (q.kind == kKindImage && q.tiled) {
+      const std::vector<int>& ord = s->flight[rid].ord;
+      SRV_HIP(hipMemcpyAsync((void*)q.order, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    }
     if (q.kind == kKindShadow) {
       // the Flight entry owns the host table until the request completes
       const std::vector<double>& cs = s->flight[rid].cs;
@@ -791,7 +833,7 @@ int mk_server_submit_image(mk_server* s, const mk_image_request* r, double* h_ou
   p.kind = kKindImage;
   p.img = *r;
   p.h_out = h_out;
-  p.bytes = align_up((size_t)r->resolution * r->resolution * sizeof(double));
+  p.bytes = image_bytes(r->resolution);
   if (p.bytes > s->arena_bytes) {
     set_error("mk_server_submit_image: %zu bytes of output exceed the %zu-byte budget", p.bytes, s->arena_bytes);
     return MK_ERR_NOMEM;

@@ -101,7 +101,9 @@ def test_served_shadow_equals_bisection_kernel():
 
 def test_budget_back_pressure_and_errors(models):
     res = 48
-    need = res * res * 8  # one output buffer (256-byte aligned already)
+    # one request: its output (256-byte aligned already) and its tile order
+    # (36 tiles x 4 bytes, aligned to 256)
+    need = res * res * 8 + 256
     with Server("mixed", hbm_budget=need, lanes=4096) as srv:
         sid = srv.add_snapshot(models[0])
         ref = _render(models[0], res, 0.5, 30.0, 230e9, "mixed")
