@@ -1,5 +1,7 @@
 // mk_render.h -- device-side parameters of the fused render kernel.
 #pragma once
+#include <vector>
+
 #include "mk_device.h"
 
 namespace mk {
@@ -10,10 +12,18 @@ struct RenderDev {
   long ntiles;
   ImageParams ip;
   double fp64_r, ring_r, ring_drdl;
-  int spin_limit;  // polls before a bounded wait gives up (paired and triple forms)
+  int spin_limit;     // polls before a bounded wait gives up (paired and triple forms)
+  const int* order;   // tiles in the order they are handed out (nullptr: row-major)
 };
 
 int fill_render_dev(const mk_render_params* p, RenderDev* rp);
+
+// The 8x8 tiles of a res x res image (row-major index) ordered by the distance
+// of the tile centre from the ring of radius 5.2 M in the image plane, nearest
+// first: roughly where a distant camera sees the photon ring, whose rays take
+// the most steps.  Handing the longest tiles out first shortens a launch's
+// tail; only the schedule changes.
+std::vector<int> ring_first_tile_order(int res, double lo, double step);
 
 // LDS budget for the staged block geometry + lookup grid (bytes)
 constexpr size_t kMaxTableLds = 64 * 1024;

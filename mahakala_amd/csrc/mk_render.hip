@@ -14,8 +14,13 @@
 // skipped.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
+#include <vector>
 
 #include "mk_device.h"
 #include "mk_internal.h"
@@ -145,9 +150,16 @@ __device__ __forceinline__ void trace_ray(const SnapView& sv, const double* meta
 __device__ __forceinline__ bool pixel_of_lane(const RenderDev& rp, int& i, int& j) {
   const unsigned nwg = gridDim.x, wg = blockIdx.x;
   const unsigned per = nwg / 8;
-  const unsigned lwg = (nwg % 8 == 0) ? (wg % 8) * per + wg / 8 : wg;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const long tile = (long)lwg * (blockDim.x >> 6) + wave;
+  long tile;
+  if (rp.order) {  // workgroups start in index order: the longest tiles first
+    const long k = (long)wg * (blockDim.x >> 6) + wave;
+    if (k >= rp.ntiles) return false;
+    tile = rp.order[k];
+  } else {
+    const unsigned lwg = (nwg % 8 == 0) ? (wg % 8) * per + wg / 8 : wg;
+    tile = (long)lwg * (blockDim.x >> 6) + wave;
+  }
   if (tile >= rp.ntiles) return false;
   const int ti = (int)(tile / rp.tiles_x), tj = (int)(tile % rp.tiles_x);
   i = ti * 8 + (lane >> 3);
@@ -476,11 +488,14 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   T* ringBt = ringBt_all + (size_t)tl * DB * CB * 64 + lane;
   int* c = ctr + 8 * tl;
   const unsigned nwg = gridDim.x, wg = blockIdx.x, per = nwg / 8;
-  const unsigned lwg = (nwg % 8 == 0) ? (wg % 8) * per + wg / 8 : wg;
-  // strided: the four tiles of a workgroup come from four bands of the image,
-  // so long tiles (rays near the photon ring) rarely share a CU; used when
-  // every workgroup is resident at once.  Otherwise adjacent tiles.
-  const long tile = strided ? (long)tl * nwg + lwg : (long)lwg * kPairTiles + tl;
+  // with a tile order, workgroups take ranks in index (start) order
+  const unsigned lwg = (!rp.order && nwg % 8 == 0) ? (wg % 8) * per + wg / 8 : wg;
+  // strided: the four tiles of a workgroup come from four quarters of the
+  // order (four bands of the image without one), so long tiles rarely share
+  // a CU; used when every workgroup is resident at once.  Otherwise adjacent
+  // ranks.
+  long tile = strided ? (long)tl * nwg + lwg : (long)lwg * kPairTiles + tl;
+  if (rp.order && tile < rp.ntiles) tile = rp.order[tile];
   int pi = 0, pj = 0;
   bool valid = tile < rp.ntiles;
   if (valid) {
@@ -648,6 +663,54 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   if (valid) img[p] = ok ? I : __builtin_nan("");
 }
 
+std::vector<int> ring_first_tile_order(int res, double lo, double step) {
+  const int tx = (res + 7) / 8, nt = tx * tx;
+  std::vector<double> d(nt);
+  for (int t = 0; t < nt; ++t) {
+    // tile centre: pixel 8 t + 3.5 at coordinate (2 i + 1) step + lo
+    const double x = (16.0 * (t / tx) + 8.0) * step + lo, y = (16.0 * (t % tx) + 8.0) * step + lo;
+    d[t] = std::fabs(std::sqrt(x * x + y * y) - 5.2);
+  }
+  std::vector<int> ord(nt);
+  for (int t = 0; t < nt; ++t) ord[t] = t;
+  std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) { return d[u] < d[v]; });
+  return ord;
+}
+
+// Device copies of tile orders, one per (device, resolution, fov), made on
+// first use and kept until the process ends (immutable, so launches in flight
+// never see them change).  Past kOrderCacheBytes, or while the stream is being
+// captured into a graph (no allocation there), launches use row-major tiles.
+namespace {
+std::mutex g_order_mu;
+std::map<std::tuple<int, int, double>, int*> g_orders;
+size_t g_order_bytes = 0;
+constexpr size_t kOrderCacheBytes = (size_t)64 << 20;
+}  // namespace
+
+static const int* render_tile_order(const RenderDev& rp, double fov, hipStream_t st) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  const auto key = std::make_tuple(dev, rp.res, fov);
+  std::lock_guard<std::mutex> lk(g_order_mu);
+  auto it = g_orders.find(key);
+  if (it != g_orders.end()) return it->second;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return nullptr;
+  const size_t bytes = (size_t)rp.ntiles * sizeof(int);
+  if (g_order_bytes + bytes > kOrderCacheBytes) return nullptr;
+  const std::vector<int> ord = ring_first_tile_order(rp.res, rp.lo, rp.step);
+  int* d = nullptr;
+  if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, ord.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  g_orders.emplace(key, d);
+  g_order_bytes += bytes;
+  return d;
+}
+
 size_t table_bytes(const SnapView& sv) {
   return (size_t)sv.nmb * kMeta * sizeof(double) + (size_t)sv.gx * sv.gy * sv.gz * sizeof(int);
 }
@@ -688,6 +751,7 @@ int fill_render_dev(const mk_render_params* p, RenderDev* rp) {
   rp->ring_r = p->ring_radius;
   rp->ring_drdl = p->ring_drdl;
   rp->spin_limit = (p->flags & MK_RENDER_WAIT_TEST) ? 1 : kSpinLimit;
+  rp->order = nullptr;
   return MK_OK;
 }
 
@@ -791,6 +855,7 @@ extern "C" int mk_render(const mk_snapshot* snap, const mk_render_params* p, dou
   int form = 0;
   rc = choose_form(sv, rp, p->precision, p->flags, &form);
   if (rc) return rc;
+  if (form != MK_RENDER_PAIRED) rp.order = render_tile_order(rp, p->fov, st);
   if (form == MK_RENDER_TRIPLE) {
     const int v = tri_variant(sv, p->precision);
     if (p->precision == 0)

@@ -503,25 +503,12 @@ size_t image_bytes(int res) {
   return image_out_bytes(res) + align_up(tiles * sizeof(int));
 }
 
-// The order in which an image's 8x8 tiles are handed out: by the distance of
-// the tile centre from the ring of radius 5.2 M in the image plane (about
-// where the photon ring of a distant camera falls, whose rays take the most
-// steps), nearest first, so that a request's longest tiles start first and
-// its last tile ends sooner.  Only the schedule changes: every ray's
-// arithmetic is the same in any order.
+// An image's tile order (ring_first_tile_order), cached per (resolution, fov).
 std::vector<int> tile_order(mk_server* s, int res, double fov, double lo, double step) {
   const auto key = std::make_pair(res, fov);
   auto it = s->orders.find(key);
   if (it != s->orders.end()) return it->second;
-  const int tx = res / 8, nt = tx * tx;
-  std::vector<double> d(nt);
-  for (int t = 0; t < nt; ++t) {
-    const double x = (16.0 * (t / tx) + 8.0) * step + lo, y = (16.0 * (t % tx) + 8.0) * step + lo;
-    d[t] = std::fabs(std::sqrt(x * x + y * y) - 5.2);
-  }
-  std::vector<int> ord(nt);
-  for (int t = 0; t < nt; ++t) ord[t] = t;
-  std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) { return d[u] < d[v]; });
+  std::vector<int> ord = ring_first_tile_order(res, lo, step);
   if (s->orders.size() >= 256) s->orders.clear();
   s->orders.emplace(key, ord);
   return ord;
