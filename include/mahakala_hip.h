@@ -177,7 +177,7 @@ typedef struct mk_render_params {
 } mk_render_params;
 
 /* mk_render_params.flags: kernel form.  By default the library runs a launch
- * of up to 4 8x8 tiles per SIMD in the triple form -- a geodesic wave, a
+ * of up to 3 8x8 tiles per SIMD in the triple form -- a geodesic wave, a
  * gather wave (block lookup, cell loads, interpolation) and an emission wave
  * per tile, with the snapshot's block tables in LDS where they fit and in
  * global memory otherwise -- and a larger one in the classic form (one wave

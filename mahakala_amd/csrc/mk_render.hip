@@ -821,8 +821,8 @@ static int choose_form(const SnapView& sv, const RenderDev& rp, int precision, i
     set_error("mk_render: no device");
     return MK_ERR_HIP;
   }
-  // triple form up to 4 tiles per SIMD (512^2 on 256 CUs)
-  if (rp.ntiles <= 16L * cus) *form = MK_RENDER_TRIPLE;
+  // triple form up to 3 tiles per SIMD (a 443^2 image on 256 CUs)
+  if (rp.ntiles <= 12L * cus) *form = MK_RENDER_TRIPLE;
   return MK_OK;
 }
 

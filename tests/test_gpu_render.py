@@ -247,11 +247,12 @@ def test_bounded_wait_failure_path(torus, form):
 
 
 def test_form_choice(torus):
-    """Launches of up to 4 tiles per SIMD (512^2 on 256 CUs) run the triple
+    """Launches of up to 3 tiles per SIMD (443^2 on 256 CUs) run the triple
     form, larger ones the classic form; a forced form is honoured."""
     for prec in ("plain64", "mixed"):
         assert _form_of(torus, images.render_params(0.5, resolution=256, precision=prec)) == "triple"
-        assert _form_of(torus, images.render_params(0.5, resolution=512, precision=prec)) == "triple"
+        assert _form_of(torus, images.render_params(0.5, resolution=384, precision=prec)) == "triple"
+        assert _form_of(torus, images.render_params(0.5, resolution=512, precision=prec)) == "classic"
         assert _form_of(torus, images.render_params(0.5, resolution=1024, precision=prec)) == "classic"
         assert _form_of(torus, images.render_params(0.5, resolution=1024, precision=prec, form="paired")) == "paired"
 
