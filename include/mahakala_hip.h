@@ -192,6 +192,9 @@ typedef struct mk_render_params {
  * wave that gives up marks its ring failed, and the tile's pixels are NaN
  * and its step counts -1).  Normal waits give up after 2^24 polls. */
 #define MK_RENDER_WAIT_TEST 256
+/* Hand tiles out in row-major order instead of photon ring first (the same
+ * image, bit for bit; only the schedule differs). */
+#define MK_RENDER_ROW_MAJOR 512
 
 /* make_image  (images.py:30-144) for a device-resident snapshot: camera,
  * geodesic, sampling, emission and transfer fused per ray; d_image

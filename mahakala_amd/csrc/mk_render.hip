@@ -855,7 +855,7 @@ extern "C" int mk_render(const mk_snapshot* snap, const mk_render_params* p, dou
   int form = 0;
   rc = choose_form(sv, rp, p->precision, p->flags, &form);
   if (rc) return rc;
-  if (form != MK_RENDER_PAIRED) rp.order = render_tile_order(rp, p->fov, st);
+  if (form != MK_RENDER_PAIRED && !(p->flags & MK_RENDER_ROW_MAJOR)) rp.order = render_tile_order(rp, p->fov, st);
   if (form == MK_RENDER_TRIPLE) {
     const int v = tri_variant(sv, p->precision);
     if (p->precision == 0)

@@ -246,6 +246,22 @@ def test_bounded_wait_failure_path(torus, form):
     np.testing.assert_array_equal(sagain, sref)
 
 
+@pytest.mark.parametrize("form", ["classic", "triple"])
+def test_tile_order_changes_only_the_schedule(torus, form):
+    """Tiles handed out photon ring first (the default) or row-major
+    (MK_RENDER_ROW_MAJOR): the same image and step counts, bit for bit, on a
+    launch of several rounds whose edge tiles are partly empty (1000^2)."""
+    out = {}
+    for flag in (0, 512):
+        p = images.render_params(0.5, camera_inclination=30, resolution=1000, precision="mixed", form=form)
+        p.flags |= flag
+        out[flag] = images.render(torus.device_snapshot, p, return_steps=True)
+    (a, sa), (b, sb) = out[0], out[512]
+    assert a.max() > 0 and np.all(np.isfinite(a))
+    np.testing.assert_array_equal(sa, sb)
+    np.testing.assert_array_equal(a, b)
+
+
 def test_form_choice(torus):
     """Launches of up to 3 tiles per SIMD (443^2 on 256 CUs) run the triple
     form, larger ones the classic form; a forced form is honoured."""
