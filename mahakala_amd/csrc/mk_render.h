@@ -19,11 +19,10 @@ struct RenderDev {
 int fill_render_dev(const mk_render_params* p, RenderDev* rp);
 
 // The 8x8 tiles of a res x res image (row-major index) ordered by the distance
-// of the tile centre from the ring of radius 5.2 M in the image plane, nearest
-// first: roughly where a distant camera sees the photon ring, whose rays take
-// the most steps.  Handing the longest tiles out first shortens a launch's
-// tail; only the schedule changes.
-std::vector<int> ring_first_tile_order(int res, double lo, double step);
+// of the tile centre from the critical curve of a distant camera (the photon
+// ring, whose rays take the most steps), nearest first.  Handing the longest
+// tiles out first shortens a launch's tail; only the schedule changes.
+std::vector<int> ring_first_tile_order(int res, double lo, double step, double a, double inc_deg);
 
 // LDS budget for the staged block geometry + lookup grid (bytes)
 constexpr size_t kMaxTableLds = 64 * 1024;

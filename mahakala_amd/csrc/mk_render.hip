@@ -663,13 +663,80 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   if (valid) img[p] = ok ? I : __builtin_nan("");
 }
 
-std::vector<int> ring_first_tile_order(int res, double lo, double step) {
+// Critical curve of a distant camera at inclination theta (Bardeen 1973):
+// spherical photon orbits of radius r in [r_pro, r_retro] appear at
+// alpha = -xi / sin(theta), beta = +-sqrt(eta + a^2 cos^2(theta) -
+// xi^2 cot^2(theta)), with xi = (r^2 (3 - r) - a^2 (r + 1)) / (a (r - 1)) and
+// eta = r^3 (4 a^2 - r (r - 3)^2) / (a^2 (r - 1)^2).  Image x is alpha and
+// image y is beta in this camera's convention.  Returned as a polar table
+// R(phi) about the curve's centre (cx, 0), kBins bins; a = 0 gives the circle
+// of radius sqrt(27).
+namespace {
+constexpr int kBins = 256;
+void critical_curve_polar(double a, double inc_deg, double* cx, double R[kBins]) {
+  const double th = std::min(std::max(inc_deg, 1.0), 179.0) * M_PI / 180.0;
+  *cx = 0.0;
+  if (std::fabs(a) < 1e-3) {
+    for (int k = 0; k < kBins; ++k) R[k] = std::sqrt(27.0);
+    return;
+  }
+  const double aa = std::fabs(a) > 0.999 ? (a > 0 ? 0.999 : -0.999) : a;
+  const double r1 = 2.0 * (1.0 + std::cos(2.0 / 3.0 * std::acos(-std::fabs(aa))));
+  const double r2 = 2.0 * (1.0 + std::cos(2.0 / 3.0 * std::acos(std::fabs(aa))));
+  const double ct = std::cos(th) / std::sin(th), c2 = std::cos(th) * std::cos(th);
+  std::vector<double> al, be;
+  const int n = 2048;
+  for (int q = 0; q <= n; ++q) {
+    const double r = std::min(r1, r2) + (std::fabs(r2 - r1)) * q / n;
+    if (std::fabs(r - 1.0) < 1e-9) continue;
+    const double xi = (r * r * (3.0 - r) - aa * aa * (r + 1.0)) / (aa * (r - 1.0));
+    const double eta = r * r * r * (4.0 * aa * aa - r * (r - 3.0) * (r - 3.0)) / (aa * aa * (r - 1.0) * (r - 1.0));
+    const double b2 = eta + aa * aa * c2 - xi * xi * ct * ct;
+    if (!(b2 >= 0.0)) continue;
+    al.push_back(-xi / std::sin(th));
+    be.push_back(std::sqrt(b2));
+  }
+  if (al.size() < 8) {
+    for (int k = 0; k < kBins; ++k) R[k] = std::sqrt(27.0);
+    return;
+  }
+  *cx = 0.5 * (*std::min_element(al.begin(), al.end()) + *std::max_element(al.begin(), al.end()));
+  std::vector<double> sum(kBins, 0.0);
+  std::vector<int> cnt(kBins, 0);
+  for (size_t q = 0; q < al.size(); ++q)
+    for (int sgn = -1; sgn <= 1; sgn += 2) {
+      const double dx = al[q] - *cx, dy = sgn * be[q];
+      int k = (int)std::floor((std::atan2(dy, dx) + M_PI) / (2.0 * M_PI) * kBins);
+      k = std::min(std::max(k, 0), kBins - 1);
+      sum[k] += std::sqrt(dx * dx + dy * dy);
+      ++cnt[k];
+    }
+  // empty bins take the nearest filled bin's radius (circularly)
+  for (int k = 0; k < kBins; ++k) {
+    if (cnt[k]) {
+      R[k] = sum[k] / cnt[k];
+      continue;
+    }
+    for (int d = 1; d < kBins; ++d) {
+      const int u = (k + d) % kBins, v = (k - d + kBins) % kBins;
+      if (cnt[u]) { R[k] = sum[u] / cnt[u]; break; }
+      if (cnt[v]) { R[k] = sum[v] / cnt[v]; break; }
+    }
+  }
+}
+}  // namespace
+
+std::vector<int> ring_first_tile_order(int res, double lo, double step, double a, double inc_deg) {
+  double cx, R[kBins];
+  critical_curve_polar(a, inc_deg, &cx, R);
   const int tx = (res + 7) / 8, nt = tx * tx;
   std::vector<double> d(nt);
   for (int t = 0; t < nt; ++t) {
     // tile centre: pixel 8 t + 3.5 at coordinate (2 i + 1) step + lo
-    const double x = (16.0 * (t / tx) + 8.0) * step + lo, y = (16.0 * (t % tx) + 8.0) * step + lo;
-    d[t] = std::fabs(std::sqrt(x * x + y * y) - 5.2);
+    const double x = (16.0 * (t / tx) + 8.0) * step + lo - cx, y = (16.0 * (t % tx) + 8.0) * step + lo;
+    int k = (int)std::floor((std::atan2(y, x) + M_PI) / (2.0 * M_PI) * kBins);
+    k = std::min(std::max(k, 0), kBins - 1);
+    d[t] = std::fabs(std::sqrt(x * x + y * y) - R[k]);
   }
   std::vector<int> ord(nt);
   for (int t = 0; t < nt; ++t) ord[t] = t;
@@ -677,21 +744,22 @@ std::vector<int> ring_first_tile_order(int res, double lo, double step) {
   return ord;
 }
 
-// Device copies of tile orders, one per (device, resolution, fov), made on
-// first use and kept until the process ends (immutable, so launches in flight
-// never see them change).  Past kOrderCacheBytes, or while the stream is being
-// captured into a graph (no allocation there), launches use row-major tiles.
+// Device copies of tile orders, one per (device, resolution, fov, spin,
+// inclination), made on first use and kept until the process ends (immutable,
+// so launches in flight never see them change).  Past kOrderCacheBytes, or
+// while the stream is being captured into a graph (no allocation there),
+// launches use row-major tiles.
 namespace {
 std::mutex g_order_mu;
-std::map<std::tuple<int, int, double>, int*> g_orders;
+std::map<std::tuple<int, int, double, double, double>, int*> g_orders;
 size_t g_order_bytes = 0;
 constexpr size_t kOrderCacheBytes = (size_t)64 << 20;
 }  // namespace
 
-static const int* render_tile_order(const RenderDev& rp, double fov, hipStream_t st) {
+static const int* render_tile_order(const RenderDev& rp, const mk_render_params* p, hipStream_t st) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-  const auto key = std::make_tuple(dev, rp.res, fov);
+  const auto key = std::make_tuple(dev, rp.res, p->fov, p->bhspin, p->inclination);
   std::lock_guard<std::mutex> lk(g_order_mu);
   auto it = g_orders.find(key);
   if (it != g_orders.end()) return it->second;
@@ -699,7 +767,7 @@ static const int* render_tile_order(const RenderDev& rp, double fov, hipStream_t
   if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return nullptr;
   const size_t bytes = (size_t)rp.ntiles * sizeof(int);
   if (g_order_bytes + bytes > kOrderCacheBytes) return nullptr;
-  const std::vector<int> ord = ring_first_tile_order(rp.res, rp.lo, rp.step);
+  const std::vector<int> ord = ring_first_tile_order(rp.res, rp.lo, rp.step, p->bhspin, p->inclination);
   int* d = nullptr;
   if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
   if (hipMemcpy(d, ord.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
@@ -855,7 +923,7 @@ extern "C" int mk_render(const mk_snapshot* snap, const mk_render_params* p, dou
   int form = 0;
   rc = choose_form(sv, rp, p->precision, p->flags, &form);
   if (rc) return rc;
-  if (form != MK_RENDER_PAIRED && !(p->flags & MK_RENDER_ROW_MAJOR)) rp.order = render_tile_order(rp, p->fov, st);
+  if (form != MK_RENDER_PAIRED && !(p->flags & MK_RENDER_ROW_MAJOR)) rp.order = render_tile_order(rp, p, st);
   if (form == MK_RENDER_TRIPLE) {
     const int v = tri_variant(sv, p->precision);
     if (p->precision == 0)

@@ -503,12 +503,16 @@ size_t image_bytes(int res) {
   return image_out_bytes(res) + align_up(tiles * sizeof(int));
 }
 
-// An image's tile order (ring_first_tile_order), cached per (resolution, fov).
+// An image's tile order: the Schwarzschild critical circle (radius sqrt(27) M)
+// for every request, cached per (resolution, fov).  The spin's own curve
+// orders a little better but costs the admitting thread about 0.3 ms per
+// 512^2 request, which the latency of a Poisson stream shows (p50 8.9 ->
+// 9.2 ms).
 std::vector<int> tile_order(mk_server* s, int res, double fov, double lo, double step) {
   const auto key = std::make_pair(res, fov);
   auto it = s->orders.find(key);
   if (it != s->orders.end()) return it->second;
-  std::vector<int> ord = ring_first_tile_order(res, lo, step);
+  std::vector<int> ord = ring_first_tile_order(res, lo, step, 0.0, 90.0);
   if (s->orders.size() >= 256) s->orders.clear();
   s->orders.emplace(key, ord);
   return ord;
