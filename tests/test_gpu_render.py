@@ -247,13 +247,15 @@ def test_bounded_wait_failure_path(torus, form):
 
 
 @pytest.mark.parametrize("form", ["classic", "triple"])
-def test_tile_order_changes_only_the_schedule(torus, form):
-    """Tiles handed out photon ring first (the default) or row-major
-    (MK_RENDER_ROW_MAJOR): the same image and step counts, bit for bit, on a
-    launch of several rounds whose edge tiles are partly empty (1000^2)."""
+@pytest.mark.parametrize("spin,inc", [(0.5, 30.0), (0.98, 85.0)])
+def test_tile_order_changes_only_the_schedule(torus, form, spin, inc):
+    """Tiles handed out photon ring first (the default; the critical curve at
+    the image's spin and inclination) or row-major (MK_RENDER_ROW_MAJOR): the
+    same image and step counts, bit for bit, on a launch of several rounds
+    whose edge tiles are partly empty (1000^2)."""
     out = {}
     for flag in (0, 512):
-        p = images.render_params(0.5, camera_inclination=30, resolution=1000, precision="mixed", form=form)
+        p = images.render_params(spin, camera_inclination=inc, resolution=1000, precision="mixed", form=form)
         p.flags |= flag
         out[flag] = images.render(torus.device_snapshot, p, return_steps=True)
     (a, sa), (b, sb) = out[0], out[512]
