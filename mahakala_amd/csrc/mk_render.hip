@@ -491,10 +491,12 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   // with a tile order, workgroups take ranks in index (start) order
   const unsigned lwg = (!rp.order && nwg % 8 == 0) ? (wg % 8) * per + wg / 8 : wg;
   // strided: the four tiles of a workgroup come from four quarters of the
-  // order (four bands of the image without one), so long tiles rarely share
-  // a CU; used when every workgroup is resident at once.  Otherwise adjacent
-  // ranks.
-  long tile = strided ? (long)tl * nwg + lwg : (long)lwg * kPairTiles + tl;
+  // order (four bands of the image without one), taken in snake order (ranks
+  // w, 2n - 1 - w, 2n + w, 4n - 1 - w), so the longest tiles share their CU
+  // with the shortest; used when every workgroup is resident at once.
+  // Otherwise adjacent ranks.
+  long tile = strided ? ((tl & 1) ? (long)(tl + 1) * nwg - 1 - lwg : (long)tl * nwg + lwg)
+                      : (long)lwg * kPairTiles + tl;
   if (rp.order && tile < rp.ntiles) tile = rp.order[tile];
   int pi = 0, pj = 0;
   bool valid = tile < rp.ntiles;
