@@ -200,35 +200,36 @@ __device__ __forceinline__ void rk4_step(const T s[8], T dt, T a, T out[8]) {
   for (int i = 0; i < 8; ++i) out[i] = s[i] + T(1.0 / 6.0) * (sum[i] + k[i]);
 }
 
-// Mixed step: stages in FP32, increment added to the FP64 state.
+// Mixed step: stages in FP32, increment added to the FP64 state.  The stage
+// slopes stay unscaled (t = s + (h/2) F is one FMA, the slope sum is
+// F1 + 2 F2 + 2 F3 + F4 and h/6 scales it once), which saves the 32 products
+// k = h F of the scaled form.
 __device__ __forceinline__ void rk4_step_mixed(const double s[8], double dt, float a, double out[8]) {
-  float sf[8], inc[8];
+  float sf[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) sf[i] = (float)s[i];
-  const float dtf = (float)dt;
-  float k[8], acc[4], t[8], sum[8];
+  const float h = (float)dt, hh = 0.5f * h, h6 = h * (1.0f / 6.0f);
+  float F[8], acc[4], t[8], sum[8];
   ks_accel<float>(sf[1], sf[2], sf[3], sf[4], sf[5], sf[6], sf[7], a, acc);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) { k[i] = dtf * sf[4 + i]; k[4 + i] = dtf * acc[i]; }
+  for (int i = 0; i < 4; ++i) { F[i] = sf[4 + i]; F[4 + i] = acc[i]; }
 #pragma unroll
-  for (int i = 0; i < 8; ++i) { sum[i] = k[i]; t[i] = sf[i] + 0.5f * k[i]; }
+  for (int i = 0; i < 8; ++i) { sum[i] = F[i]; t[i] = sf[i] + hh * F[i]; }
   ks_accel<float>(t[1], t[2], t[3], t[4], t[5], t[6], t[7], a, acc);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) { k[i] = dtf * t[4 + i]; k[4 + i] = dtf * acc[i]; }
+  for (int i = 0; i < 4; ++i) { F[i] = t[4 + i]; F[4 + i] = acc[i]; }
 #pragma unroll
-  for (int i = 0; i < 8; ++i) { sum[i] += 2.0f * k[i]; t[i] = sf[i] + 0.5f * k[i]; }
+  for (int i = 0; i < 8; ++i) { sum[i] += 2.0f * F[i]; t[i] = sf[i] + hh * F[i]; }
   ks_accel<float>(t[1], t[2], t[3], t[4], t[5], t[6], t[7], a, acc);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) { k[i] = dtf * t[4 + i]; k[4 + i] = dtf * acc[i]; }
+  for (int i = 0; i < 4; ++i) { F[i] = t[4 + i]; F[4 + i] = acc[i]; }
 #pragma unroll
-  for (int i = 0; i < 8; ++i) { sum[i] += 2.0f * k[i]; t[i] = sf[i] + k[i]; }
+  for (int i = 0; i < 8; ++i) { sum[i] += 2.0f * F[i]; t[i] = sf[i] + h * F[i]; }
   ks_accel<float>(t[1], t[2], t[3], t[4], t[5], t[6], t[7], a, acc);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) { k[i] = dtf * t[4 + i]; k[4 + i] = dtf * acc[i]; }
+  for (int i = 0; i < 4; ++i) { F[i] = t[4 + i]; F[4 + i] = acc[i]; }
 #pragma unroll
-  for (int i = 0; i < 8; ++i) inc[i] = (1.0f / 6.0f) * (sum[i] + k[i]);
-#pragma unroll
-  for (int i = 0; i < 8; ++i) out[i] = s[i] + (double)inc[i];
+  for (int i = 0; i < 8; ++i) out[i] = s[i] + (double)(h6 * (sum[i] + F[i]));
 }
 
 // geodesics.py:284-291
