@@ -497,6 +497,14 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   // Otherwise adjacent ranks.
   long tile = strided ? ((tl & 1) ? (long)(tl + 1) * nwg - 1 - lwg : (long)tl * nwg + lwg)
                       : (long)lwg * kPairTiles + tl;
+  // Strided with a tile order: the three waves of the tile from the first
+  // quarter (the longest tiles) issue at a raised priority, so where the waves
+  // of a SIMD compete it goes first; the other tiles have slack until it
+  // ends, and the waits of every wave sleep, so they still run.  (Four levels,
+  // one per quarter, are 1 % slower in mixed and 2.5 % faster in FP64.)
+  // s_setprio is a scalar instruction: it takes effect whatever the exec
+  // mask, so the condition is made uniform (an SGPR and a scalar branch).
+  if (strided && rp.order && __builtin_amdgcn_readfirstlane(tl) == 0) __builtin_amdgcn_s_setprio(1);
   if (rp.order && tile < rp.ntiles) tile = rp.order[tile];
   int pi = 0, pj = 0;
   bool valid = tile < rp.ntiles;
