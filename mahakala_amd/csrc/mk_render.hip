@@ -458,7 +458,7 @@ __device__ __forceinline__ int wait_round(int* produced, int* done, int t, int* 
 template <bool MIXED, bool LDS, int DB_>
 __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, RenderDev rp, double* __restrict__ img,
                                                                   int32_t* __restrict__ steps, int tab_doubles,
-                                                                  int strided) {
+                                                                  int nres) {
   using T = typename std::conditional<MIXED, float, double>::type;
   using R = TriRings<MIXED, DB_>;
   constexpr int DA = R::kDepthA, DB = R::kDepthB, CB = R::kCompB;
@@ -490,13 +490,19 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   const unsigned nwg = gridDim.x, wg = blockIdx.x, per = nwg / 8;
   // with a tile order, workgroups take ranks in index (start) order
   const unsigned lwg = (!rp.order && nwg % 8 == 0) ? (wg % 8) * per + wg / 8 : wg;
-  // strided: the four tiles of a workgroup come from four quarters of the
-  // order (four bands of the image without one), taken in snake order (ranks
-  // w, 2n - 1 - w, 2n + w, 4n - 1 - w), so the longest tiles share their CU
-  // with the shortest; used when every workgroup is resident at once.
-  // Otherwise adjacent ranks.
-  long tile = strided ? ((tl & 1) ? (long)(tl + 1) * nwg - 1 - lwg : (long)tl * nwg + lwg)
-                      : (long)lwg * kPairTiles + tl;
+  // Workgroups lwg < n (n = nres) are strided: their four tiles come from four
+  // quarters of ranks [0, 4n) of the order (four bands of the image without
+  // one), taken in snake order (ranks w, 2n - 1 - w, 2n + w, 4n - 1 - w), so
+  // the longest tiles share their CU with the shortest.  The host sets n to
+  // the whole grid when every workgroup is resident at once, to the resident
+  // count when a tile order spreads over at most two waves of workgroups (the
+  // first wave then holds one long tile per CU, and the second, the shortest
+  // tiles, starts as CUs free), else 0.  Workgroups from n on take the rest of
+  // the ranks in order, four adjacent ones each.
+  const unsigned n = (unsigned)nres;
+  const bool first = lwg < n;
+  long tile = first ? ((tl & 1) ? (long)(tl + 1) * n - 1 - lwg : (long)tl * n + lwg)
+                    : 4L * n + (long)(lwg - n) * kPairTiles + tl;
   // Strided with a tile order: the three waves of the tile from the first
   // quarter (the longest tiles) issue at a raised priority, so where the waves
   // of a SIMD compete it goes first; the other tiles have slack until it
@@ -504,7 +510,7 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   // one per quarter, are 1 % slower in mixed and 2.5 % faster in FP64.)
   // s_setprio is a scalar instruction: it takes effect whatever the exec
   // mask, so the condition is made uniform (an SGPR and a scalar branch).
-  if (strided && rp.order && __builtin_amdgcn_readfirstlane(tl) == 0) __builtin_amdgcn_s_setprio(1);
+  if (first && rp.order && __builtin_amdgcn_readfirstlane(tl) == 0) __builtin_amdgcn_s_setprio(1);
   if (rp.order && tile < rp.ntiles) tile = rp.order[tile];
   int pi = 0, pj = 0;
   bool valid = tile < rp.ntiles;
@@ -849,7 +855,7 @@ static int launch_tri(const SnapView& sv, const RenderDev& rp, double* d_image, 
   long nwg = (rp.ntiles + kPairTiles - 1) / kPairTiles;
   nwg = (nwg + 7) / 8 * 8;
   const size_t lds = tab + tri_lds_extra<MIXED, DB_>();
-  // strided tiles when every workgroup of the launch is resident at once
+  // strided tiles for the workgroups resident at once (see k_render_tri)
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -859,9 +865,10 @@ static int launch_tri(const SnapView& sv, const RenderDev& rp, double* d_image, 
     set_error("mk_render: occupancy query failed");
     return MK_ERR_HIP;
   }
-  const int strided = nwg <= (long)per_cu * cus;
+  const long R = (long)per_cu * cus;
+  const int nres = nwg <= R ? (int)nwg : (rp.order && nwg <= 2 * R ? (int)R : 0);
   k_render_tri<MIXED, LDS, DB_><<<dim3((unsigned)nwg), dim3(192 * kPairTiles), lds, st>>>(
-      sv, rp, d_image, d_steps, (int)(tab / sizeof(double)), strided);
+      sv, rp, d_image, d_steps, (int)(tab / sizeof(double)), nres);
   MK_LAUNCH_CHECK();
   return MK_OK;
 }
