@@ -906,8 +906,10 @@ static int choose_form(const SnapView& sv, const RenderDev& rp, int precision, i
     set_error("mk_render: no device");
     return MK_ERR_HIP;
   }
-  // triple form up to 3 tiles per SIMD (a 443^2 image on 256 CUs)
-  if (rp.ntiles <= 12L * cus) *form = MK_RENDER_TRIPLE;
+  // triple form up to 15.5 tiles per CU in mixed precision (a 496^2 image on
+  // 256 CUs) and 11 in FP64 (424^2); the classic form beyond, where every
+  // SIMD holds three waves of whole rays
+  if (2 * rp.ntiles <= (precision == 1 ? 31L : 22L) * cus) *form = MK_RENDER_TRIPLE;
   return MK_OK;
 }
 

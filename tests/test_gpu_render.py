@@ -267,11 +267,14 @@ def test_tile_order_changes_only_the_schedule(torus, form, spin, inc):
 
 
 def test_form_choice(torus):
-    """Launches of up to 3 tiles per SIMD (443^2 on 256 CUs) run the triple
-    form, larger ones the classic form; a forced form is honoured."""
+    """Launches of up to 15.5 tiles per CU in mixed precision (496^2 on 256
+    CUs) and 11 in FP64 (424^2) run the triple form, larger ones the classic
+    form; a forced form is honoured."""
     for prec in ("plain64", "mixed"):
         assert _form_of(torus, images.render_params(0.5, resolution=256, precision=prec)) == "triple"
         assert _form_of(torus, images.render_params(0.5, resolution=384, precision=prec)) == "triple"
+        assert _form_of(torus, images.render_params(0.5, resolution=448, precision=prec)) == (
+            "triple" if prec == "mixed" else "classic")
         assert _form_of(torus, images.render_params(0.5, resolution=512, precision=prec)) == "classic"
         assert _form_of(torus, images.render_params(0.5, resolution=1024, precision=prec)) == "classic"
         assert _form_of(torus, images.render_params(0.5, resolution=1024, precision=prec, form="paired")) == "paired"
