@@ -536,7 +536,15 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
     bool ok = true;
     int t = 0, freed = 0;  // freed: ring A rounds the gather wave has read (last value seen)
     while (__any(alive)) {
-      if (t - DA + 1 > freed) {
+      // Slot t % DA is free once the gather wave has read round t - DA.  FP64
+      // loads that count here without waiting and checks it after the RK4
+      // step: a wait at the loop head would also wait on this wave's slot
+      // writes of the round before (-2 to -4 % in FP64; in mixed it was
+      // slower for launches of several workgroup waves).
+      int polled = 0;
+      if constexpr (!MIXED) {
+        polled = __hip_atomic_load(c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      } else if (t - DA + 1 > freed) {
         freed = wait_value(c + 1, t - DA + 1, rp.spin_limit);
         if (freed < 0) {
           ok = false;
@@ -545,7 +553,11 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
       }
       double* out = ringA + (size_t)(t % DA) * kPairComp * 64;
       if (alive && (k >= rp.N || dt == 0.0)) alive = false;
-      double nw[8];
+      // mixed steps the state in place (the step functions write their output
+      // only after reading the state; a ray that stops is not used again):
+      // -1.3 %; in FP64 the copy is faster
+      double nw_[8];
+      double* nw = MIXED ? s : nw_;
       if (alive) {
         if constexpr (!MIXED) {
           rk4_step<double>(s, dt, rp.a, nw);
@@ -558,6 +570,21 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
       // release the previous round here: its slot writes were issued a whole
       // RK4 step ago, so the release fence does not wait on them
       if (t > 0) publish(c + 0, t);
+      if constexpr (!MIXED) {
+        const int seen = __builtin_amdgcn_readfirstlane(polled);
+        if (t - DA + 1 > freed) {
+          if (seen >= t - DA + 1) {
+            freed = seen;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+          } else {
+            freed = wait_value(c + 1, t - DA + 1, rp.spin_limit);
+            if (freed < 0) {
+              ok = false;
+              break;
+            }
+          }
+        }
+      }
       bool wrote = false;
       if (alive) {
         double rn, dtn;
@@ -579,8 +606,10 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
             wrote = true;
           }
           if (zone) drdl = (rn - r) / dt;
+          if constexpr (!MIXED) {
 #pragma unroll
-          for (int q = 0; q < 8; ++q) s[q] = nw[q];
+            for (int q = 0; q < 8; ++q) s[q] = nw[q];
+          }
           dt = dtn;
           r = rn;
           if (++k >= rp.N) alive = false;
