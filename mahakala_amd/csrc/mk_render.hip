@@ -454,8 +454,11 @@ __device__ __forceinline__ int wait_round(int* produced, int* done, int t, int* 
 
 // LDS: the block tables are staged in LDS; otherwise (a snapshot whose tables
 // exceed the LDS budget) they are read from global memory, and the gather
-// wave keeps each ray's last block record in registers.
-template <bool MIXED, bool LDS, int DB_>
+// wave keeps each ray's last block record in registers.  ZONE: the mixed
+// geodesic wave can take FP64 substeps in the zone (rp.fp64_r, rp.ring_r);
+// without it the zone code is compiled out (the same arithmetic when both
+// radii are 0).
+template <bool MIXED, bool LDS, int DB_, bool ZONE>
 __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, RenderDev rp, double* __restrict__ img,
                                                                   int32_t* __restrict__ steps, int tab_doubles,
                                                                   int nres) {
@@ -532,7 +535,7 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
       dt = step_rule(r, rp.rEH, rp.div, rp.tol);
       alive = true;
     }
-    const bool zone = MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0);
+    const bool zone = ZONE && MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0);
     bool ok = true;
     int t = 0, freed = 0;  // freed: ring A rounds the gather wave has read (last value seen)
     while (__any(alive)) {
@@ -889,15 +892,25 @@ static int launch_tri(const SnapView& sv, const RenderDev& rp, double* d_image, 
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_tri<MIXED, LDS, DB_>, 192 * kPairTiles, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_tri<MIXED, LDS, DB_, MIXED>, 192 * kPairTiles,
+                                                     lds);
   if (e != hipSuccess) {
     set_error("mk_render: occupancy query failed");
     return MK_ERR_HIP;
   }
   const long R = (long)per_cu * cus;
   const int nres = nwg <= R ? (int)nwg : (rp.order && nwg <= 2 * R ? (int)R : 0);
-  k_render_tri<MIXED, LDS, DB_><<<dim3((unsigned)nwg), dim3(192 * kPairTiles), lds, st>>>(
-      sv, rp, d_image, d_steps, (int)(tab / sizeof(double)), nres);
+  const int tabd = (int)(tab / sizeof(double));
+  // mixed without the zone: the zone-free kernel where every workgroup is
+  // resident at once (-2 to -3 % at 192^2-256^2; in launches of several
+  // workgroup waves it measured 3 % slower at some sizes, so those keep the
+  // other build)
+  if (MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0 || nwg > R))
+    k_render_tri<MIXED, LDS, DB_, MIXED><<<dim3((unsigned)nwg), dim3(192 * kPairTiles), lds, st>>>(sv, rp, d_image,
+                                                                                                    d_steps, tabd, nres);
+  else
+    k_render_tri<MIXED, LDS, DB_, false><<<dim3((unsigned)nwg), dim3(192 * kPairTiles), lds, st>>>(sv, rp, d_image,
+                                                                                                    d_steps, tabd, nres);
   MK_LAUNCH_CHECK();
   return MK_OK;
 }
