@@ -540,12 +540,13 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
     int t = 0, freed = 0;  // freed: ring A rounds the gather wave has read (last value seen)
     while (__any(alive)) {
       // Slot t % DA is free once the gather wave has read round t - DA.  FP64
-      // loads that count here without waiting and checks it after the RK4
-      // step: a wait at the loop head would also wait on this wave's slot
-      // writes of the round before (-2 to -4 % in FP64; in mixed it was
-      // slower for launches of several workgroup waves).
+      // and the zone-free mixed build load that count here without waiting
+      // and check it after the RK4 step: a wait at the loop head would also
+      // wait on this wave's slot writes of the round before (-2 to -4 % in
+      // FP64, -1 % in one-wave mixed launches; the mixed zone build, which
+      // also runs launches of several workgroup waves, measured slower).
       int polled = 0;
-      if constexpr (!MIXED) {
+      if constexpr (!MIXED || !ZONE) {
         polled = __hip_atomic_load(c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       } else if (t - DA + 1 > freed) {
         freed = wait_value(c + 1, t - DA + 1, rp.spin_limit);
@@ -573,7 +574,7 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
       // release the previous round here: its slot writes were issued a whole
       // RK4 step ago, so the release fence does not wait on them
       if (t > 0) publish(c + 0, t);
-      if constexpr (!MIXED) {
+      if constexpr (!MIXED || !ZONE) {
         const int seen = __builtin_amdgcn_readfirstlane(polled);
         if (t - DA + 1 > freed) {
           if (seen >= t - DA + 1) {
