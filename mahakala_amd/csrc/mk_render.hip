@@ -509,11 +509,21 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
   // Strided with a tile order: the three waves of the tile from the first
   // quarter (the longest tiles) issue at a raised priority, so where the waves
   // of a SIMD compete it goes first; the other tiles have slack until it
-  // ends, and the waits of every wave sleep, so they still run.  (Four levels,
-  // one per quarter, are 1 % slower in mixed and 2.5 % faster in FP64.)
+  // ends, and the waits of every wave sleep, so they still run.  FP64 uses one
+  // level per quarter (1-3 % faster there than one raised level; in mixed
+  // 1 % slower).
   // s_setprio is a scalar instruction: it takes effect whatever the exec
   // mask, so the condition is made uniform (an SGPR and a scalar branch).
-  if (first && rp.order && __builtin_amdgcn_readfirstlane(tl) == 0) __builtin_amdgcn_s_setprio(1);
+  if (first && rp.order) {
+    const int q = __builtin_amdgcn_readfirstlane(tl);
+    if constexpr (MIXED) {
+      if (q == 0) __builtin_amdgcn_s_setprio(1);
+    } else {  // FP64: one level per quarter
+      if (q == 0) __builtin_amdgcn_s_setprio(3);
+      else if (q == 1) __builtin_amdgcn_s_setprio(2);
+      else if (q == 2) __builtin_amdgcn_s_setprio(1);
+    }
+  }
   if (rp.order && tile < rp.ntiles) tile = rp.order[tile];
   int pi = 0, pj = 0;
   bool valid = tile < rp.ntiles;
