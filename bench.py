@@ -375,7 +375,7 @@ def main():
             out["speedup_mixed_over_plain"] = round(pw / results["mixed"]["wall"], 3)
         if "mixed-zone" in results:
             zw = results["mixed-zone"]["wall"]
-            out["mixed_zone"] = {"what": "FP64 RK4 substeps inside r < 4 M and near the photon ring",
+            out["mixed_zone"] = {"what": "FP64 RK4 steps for an 8x8 tile's rays while any of them is inside r < 4 M or near the photon ring",
                                  "value": round(n * args.steps / zw, 1), "unit": "rays/s",
                                  "ms_per_image": round(float(np.mean(results["mixed-zone"]["ms"])), 4),
                                  "speedup_over_plain": round(pw / zw, 3),

@@ -167,9 +167,10 @@ typedef struct mk_render_params {
   double tol;                 /* horizon tolerance (1e-4)                   */
   double fallback_pitch;      /* pi/3                                       */
   double sigma_cut;           /* 100                                        */
-  double fp64_radius;         /* mixed: FP64 substeps where r < this        */
+  double fp64_radius;         /* mixed: FP64 steps where r < this           */
   double ring_radius;         /* mixed: FP64 where r < this and             */
-  double ring_drdl;           /*        |dr/dlambda| < this                 */
+  double ring_drdl;           /*        |dr/dlambda| < this; an 8x8 tile's  */
+                              /* rays step in FP64 while any ray is there   */
   int32_t resolution;         /* pixels per side                            */
   int32_t max_nsteps;         /* N (10000)                                  */
   int32_t precision;          /* 0 = plain FP64, 1 = mixed FP32/FP64        */

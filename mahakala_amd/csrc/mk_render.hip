@@ -103,7 +103,7 @@ __device__ __forceinline__ void trace_ray(const SnapView& sv, const double* meta
       if constexpr (!MIXED) {
         rk4_step<double>(s, dt, rp.a, nw);
       } else {
-        const bool fp64 = zone && ((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
+        const bool fp64 = zone && __any((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
         if (fp64) rk4_step<double>(s, dt, rp.a, nw);
         else rk4_step_mixed(s, dt, (float)rp.a, nw);
       }
@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(128 * kPairTiles) k_render_pair(SnapView sv, R
           if constexpr (!MIXED) {
             rk4_step<double>(s, dt, rp.a, nw);
           } else {
-            const bool fp64 = zone && ((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
+            const bool fp64 = zone && __any((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
             if (fp64) rk4_step<double>(s, dt, rp.a, nw);
             else rk4_step_mixed(s, dt, (float)rp.a, nw);
           }
@@ -576,7 +576,7 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
         if constexpr (!MIXED) {
           rk4_step<double>(s, dt, rp.a, nw);
         } else {
-          const bool fp64 = zone && ((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
+          const bool fp64 = zone && __any((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
           if (fp64) rk4_step<double>(s, dt, rp.a, nw);
           else rk4_step_mixed(s, dt, (float)rp.a, nw);
         }

@@ -22,8 +22,9 @@ PRECISIONS = {"plain64": 0, "fp64": 0, "mixed": 1, "mixed-zone": 1}
 # Mixed precision keeps in FP64 the ray state and its RK4 accumulation, the
 # step-size / termination rule, the camera-ray null normalisation and the
 # optical-depth / intensity accumulation; the RK4 right-hand sides, sampling
-# and emission run in FP32.  "mixed-zone" also takes the RK4 substeps in FP64
-# inside r < 4 M and near the photon ring (r < 5 M with |dr/dlambda| < 0.1).
+# and emission run in FP32.  "mixed-zone" also takes the RK4 steps in FP64
+# inside r < 4 M and near the photon ring (r < 5 M with |dr/dlambda| < 0.1):
+# an 8x8 tile's rays step in FP64 while any of them is in that zone.
 # On the 256^2 torus image the zone changes the total flux by < 1e-7
 # relative and costs ~30% in time, so plain "mixed" leaves it off.
 ZONES = {"mixed": (0.0, 0.0, 0.0), "mixed-zone": (4.0, 5.0, 0.1)}
