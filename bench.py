@@ -286,9 +286,9 @@ def main():
         if form.value == images.FORMS["triple"]:
             # the workload's 512-block tables are staged in LDS; ring B holds
             # 3 rounds in mixed, 2 in FP64; the workload's 1024 tiles make a
-            # one-wave launch, which runs without the zone code unless the
-            # preset has a zone
-            zone = "true" if m == "true" and (par[p].fp64_radius > 0 or par[p].ring_radius > 0) else "false"
+            # one-wave launch, which runs the zone-free build (0) unless the
+            # preset has a zone (1)
+            zone = 1 if m == "true" and (par[p].fp64_radius > 0 or par[p].ring_radius > 0) else 0
             return f"k_render_tri<{m}, true, {3 if m == 'true' else 2}, {zone}>"
         if form.value == images.FORMS["paired"]:
             return f"k_render_pair<{m}>"

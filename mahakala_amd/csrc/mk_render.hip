@@ -454,11 +454,13 @@ __device__ __forceinline__ int wait_round(int* produced, int* done, int t, int* 
 
 // LDS: the block tables are staged in LDS; otherwise (a snapshot whose tables
 // exceed the LDS budget) they are read from global memory, and the gather
-// wave keeps each ray's last block record in registers.  ZONE: the mixed
-// geodesic wave can take FP64 substeps in the zone (rp.fp64_r, rp.ring_r);
-// without it the zone code is compiled out (the same arithmetic when both
-// radii are 0).
-template <bool MIXED, bool LDS, int DB_, bool ZONE>
+// wave keeps each ray's last block record in registers.  ZONE (mixed): 0 the
+// zone code is compiled out; 1 the geodesic wave takes FP64 steps while any
+// of its rays is in the zone (rp.fp64_r, rp.ring_r), as the other forms do;
+// 2 only for launches without a zone, where that test never fires: the zone
+// code in the loop layout that measured fastest for launches of several
+// workgroup waves (builds 0 and 1 were 5-9 % slower there, same results).
+template <bool MIXED, bool LDS, int DB_, int ZONE>
 __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, RenderDev rp, double* __restrict__ img,
                                                                   int32_t* __restrict__ steps, int tab_doubles,
                                                                   int nres) {
@@ -545,7 +547,7 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
       dt = step_rule(r, rp.rEH, rp.div, rp.tol);
       alive = true;
     }
-    const bool zone = ZONE && MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0);
+    const bool zone = ZONE != 0 && MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0);
     bool ok = true;
     int t = 0, freed = 0;  // freed: ring A rounds the gather wave has read (last value seen)
     while (__any(alive)) {
@@ -556,7 +558,7 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
       // FP64, -1 % in one-wave mixed launches; the mixed zone build, which
       // also runs launches of several workgroup waves, measured slower).
       int polled = 0;
-      if constexpr (!MIXED || !ZONE) {
+      if constexpr (!MIXED || ZONE == 0) {
         polled = __hip_atomic_load(c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       } else if (t - DA + 1 > freed) {
         freed = wait_value(c + 1, t - DA + 1, rp.spin_limit);
@@ -576,7 +578,9 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
         if constexpr (!MIXED) {
           rk4_step<double>(s, dt, rp.a, nw);
         } else {
-          const bool fp64 = zone && __any((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
+          bool fp64;
+          if constexpr (ZONE == 2) fp64 = zone && ((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
+          else fp64 = zone && __any((r < rp.fp64_r) || (r < rp.ring_r && __builtin_fabs(drdl) < rp.ring_drdl));
           if (fp64) rk4_step<double>(s, dt, rp.a, nw);
           else rk4_step_mixed(s, dt, (float)rp.a, nw);
         }
@@ -584,7 +588,7 @@ __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, Re
       // release the previous round here: its slot writes were issued a whole
       // RK4 step ago, so the release fence does not wait on them
       if (t > 0) publish(c + 0, t);
-      if constexpr (!MIXED || !ZONE) {
+      if constexpr (!MIXED || ZONE == 0) {
         const int seen = __builtin_amdgcn_readfirstlane(polled);
         if (t - DA + 1 > freed) {
           if (seen >= t - DA + 1) {
@@ -903,8 +907,8 @@ static int launch_tri(const SnapView& sv, const RenderDev& rp, double* d_image, 
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_tri<MIXED, LDS, DB_, MIXED>, 192 * kPairTiles,
-                                                     lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_tri<MIXED, LDS, DB_, MIXED ? 1 : 0>,
+                                                     192 * kPairTiles, lds);
   if (e != hipSuccess) {
     set_error("mk_render: occupancy query failed");
     return MK_ERR_HIP;
@@ -912,16 +916,16 @@ static int launch_tri(const SnapView& sv, const RenderDev& rp, double* d_image, 
   const long R = (long)per_cu * cus;
   const int nres = nwg <= R ? (int)nwg : (rp.order && nwg <= 2 * R ? (int)R : 0);
   const int tabd = (int)(tab / sizeof(double));
-  // mixed without the zone: the zone-free kernel where every workgroup is
-  // resident at once (-2 to -3 % at 192^2-256^2; in launches of several
-  // workgroup waves it measured 3 % slower at some sizes, so those keep the
-  // other build)
-  if (MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0 || nwg > R))
-    k_render_tri<MIXED, LDS, DB_, MIXED><<<dim3((unsigned)nwg), dim3(192 * kPairTiles), lds, st>>>(sv, rp, d_image,
-                                                                                                    d_steps, tabd, nres);
+  // mixed: the zone build for a zone; without one, the zone-free build where
+  // every workgroup is resident at once (-2 to -3 % at 192^2-256^2) and
+  // build 2 in launches of several workgroup waves (see k_render_tri)
+  const dim3 grid((unsigned)nwg), blk(192 * kPairTiles);
+  if (MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0))
+    k_render_tri<MIXED, LDS, DB_, MIXED ? 1 : 0><<<grid, blk, lds, st>>>(sv, rp, d_image, d_steps, tabd, nres);
+  else if (MIXED && nwg > R)
+    k_render_tri<MIXED, LDS, DB_, MIXED ? 2 : 0><<<grid, blk, lds, st>>>(sv, rp, d_image, d_steps, tabd, nres);
   else
-    k_render_tri<MIXED, LDS, DB_, false><<<dim3((unsigned)nwg), dim3(192 * kPairTiles), lds, st>>>(sv, rp, d_image,
-                                                                                                    d_steps, tabd, nres);
+    k_render_tri<MIXED, LDS, DB_, 0><<<grid, blk, lds, st>>>(sv, rp, d_image, d_steps, tabd, nres);
   MK_LAUNCH_CHECK();
   return MK_OK;
 }
