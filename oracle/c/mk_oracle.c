@@ -156,6 +156,17 @@ static double step_rule(const double* x, double a, double rEH, double div, doubl
     return dt;
 }
 
+/* Relative distance of a step-rule magnitude m = |r - r_EH| from the nearest
+ * freeze threshold (tol or 1500), infinite for NaN: a ray whose stop
+ * decision differs between two FP64 evaluations of the same trajectory must
+ * have met a candidate within rounding of a threshold. */
+static double threshold_margin(const double* x, double a, double rEH, double tol) {
+    double m = fabs(radius_cal(x, a) - rEH);
+    if (isnan(m)) return INFINITY;
+    double d1 = fabs(m - tol) / tol, d2 = fabs(m - 1500.0) / 1500.0;
+    return d1 < d2 ? d1 : d2;
+}
+
 /* ------------------------------------------------------------------ camera */
 static void image_to_bh(double x, double y, double z, double ci, double si, double d, double* o) {
     o[0] = -y * ci + z * si + d * si;
@@ -233,6 +244,8 @@ typedef struct {
     int K;            /* first frozen row, N if never */
     double fin[8];    /* state of the ray's last emitted row (row min(K, N-1)) */
     double sel_r;     /* select_photons_integrator value for this ray */
+    double m_rej;     /* threshold_margin of the candidate that froze the ray */
+    double m_acc;     /* threshold_margin of the last accepted state */
 } ray_out_t;
 
 /* Row n of the reference's history is (s_n, dt_n); a ray frozen at row K
@@ -246,6 +259,7 @@ static void integrate_one(const double* s0, int N, double div, double tol, doubl
     double dt = step_rule(s, a, rEH, div, tol);
     double best = -INFINITY, r_prev = NAN, r_at_best_prev = NAN;
     int besti = -1, K = N;
+    o->m_rej = INFINITY;
     for (int n = 0; n < N; ++n) {
         double r_here = radius_cal(s, a);
         int frozen = (dt == 0.0);
@@ -254,6 +268,7 @@ static void integrate_one(const double* s0, int N, double div, double tol, doubl
             rk4(s, dt, a, nw);
             dtn = step_rule(nw, a, rEH, div, tol);
             frozen = (dtn == 0.0);
+            if (frozen) o->m_rej = threshold_margin(nw, a, rEH, tol);
         }
         double row_dt = frozen ? 0.0 : dt;
         if (row_dt > best) { best = row_dt; besti = n; r_at_best_prev = r_prev; }
@@ -265,12 +280,15 @@ static void integrate_one(const double* s0, int N, double div, double tol, doubl
     }
     o->K = K;
     memcpy(o->fin, s, sizeof s);
+    o->m_acc = K >= N ? INFINITY : threshold_margin(s, a, rEH, tol);
+    if (K >= N) o->m_rej = INFINITY;
     o->sel_r = (besti >= 1) ? r_at_best_prev : radius_cal(s, a);
 }
 
-/* final-state integration: K per ray and the last emitted state */
+/* final-state integration: K per ray and the last emitted state; margin
+ * (NULL: not written) 2 per ray as in mko_render */
 void mko_integrate_final(const double* s0, long npx, int N, double div, double tol, double a,
-                         int* K, double* fin, double* sel_r, int nthreads) {
+                         int* K, double* fin, double* sel_r, double* margin, int nthreads) {
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
     for (long p = 0; p < npx; ++p) {
         ray_out_t o;
@@ -278,6 +296,7 @@ void mko_integrate_final(const double* s0, long npx, int N, double div, double t
         if (K) K[p] = o.K;
         if (fin) memcpy(fin + 8 * p, o.fin, sizeof o.fin);
         if (sel_r) sel_r[p] = o.sel_r;
+        if (margin) { margin[2 * p] = o.m_rej; margin[2 * p + 1] = o.m_acc; }
     }
 }
 
@@ -520,12 +539,21 @@ static void emission(const snap_t* sn, const double* st, double a, const img_par
 
 /* Whole image: camera -> geodesic -> sampling -> emission -> transfer.
  * pixels p0 + q * stride, q < count, of the res x res grid; I (count),
- * accepted steps per ray. */
+ * accepted steps per ray.
+ * force_steps (NULL: the reference's rule) runs exactly force_steps[q]
+ * accepted steps where the ray stays finite: a candidate the freeze rule
+ * would reject is accepted with the unclamped step -(r - r_EH)/div, and the
+ * ray stops when the count is reached.  Used to compare an image whose stop
+ * decisions differ by rounding at a threshold with the same step counts.
+ * margin (NULL: not written, 2 per ray): threshold_margin of the rejected
+ * candidate that froze the ray, and of the ray's last accepted state
+ * (infinite where the ray ran into the step cap). */
 void mko_render(const float* data, long nmb, int n, const double* lo, const double* hi, const double* v0,
                 const double* dx, const int* grid, int G, const double* g0, double gh,
                 double a, double inc, double dist, double fov, int res, int max_nsteps,
                 double fluid_gamma, double r_high, double mass_scale, double M_bh, double nu_obs,
-                long p0, long count, long stride, double* I_out, int* steps_out, int nthreads) {
+                long p0, long count, long stride, double* I_out, int* steps_out, const int* force_steps,
+                double* margin, int nthreads) {
     snap_t sn = {data, nmb, n, lo, hi, v0, dx, grid, G, {g0[0], g0[1], g0[2]}, gh};
     double L_unit = GNEWT * M_bh / (CL * CL);
     double dens_unit = mass_scale / (L_unit * L_unit * L_unit);
@@ -542,13 +570,20 @@ void mko_render(const float* data, long nmb, int n, const double* lo, const doub
                    sx, sv);
         nullify(sx, sv, a, s);
         double dt = step_rule(s, a, rEH, div, tol);
-        double I = 0.0, P = 1.0;
+        const int force = force_steps ? force_steps[q] : -1;
+        if (force > 0 && dt == 0.0) dt = -(radius_cal(s, a) - rEH) / div;
+        double I = 0.0, P = 1.0, m_rej = INFINITY;
         int steps = 0;
         for (int k = 0; k < max_nsteps; ++k) {
-            if (dt == 0.0) break;
+            if (dt == 0.0 || dt != dt) break;
+            if (force >= 0 && steps >= force) break;
             rk4(s, dt, a, nw);
             double dtn = step_rule(nw, a, rEH, div, tol);
-            if (dtn == 0.0) break;
+            if (dtn == 0.0) {
+                m_rej = threshold_margin(nw, a, rEH, tol);
+                if (force >= 0 && steps + 1 < force) dtn = -(radius_cal(nw, a) - rEH) / div;
+                if (force < 0 || steps + 1 > force || dtn != dtn) break;
+            }
             ++steps;
             if (k + 1 <= max_nsteps - 1) {
                 double jv, av;
@@ -562,6 +597,11 @@ void mko_render(const float* data, long nmb, int n, const double* lo, const doub
         }
         I_out[q] = I;
         if (steps_out) steps_out[q] = steps;
+        if (margin) {
+            int capped = steps >= max_nsteps;
+            margin[2 * q] = capped ? INFINITY : m_rej;
+            margin[2 * q + 1] = capped ? INFINITY : threshold_margin(s, a, rEH, tol);
+        }
     }
 }
 

@@ -37,7 +37,7 @@ def lib():
         L.mko_camera_grid.argtypes = [C.c_double] * 5 + [C.c_int, _dp]
         L.mko_camera_pixels.argtypes = [C.c_double] * 3 + [_dp, _dp, C.c_long, _dp]
         L.mko_integrate_final.argtypes = [_dp, C.c_long, C.c_int, C.c_double, C.c_double, C.c_double,
-                                          _ip, _dp, _dp, C.c_int]
+                                          _ip, _dp, _dp, C.c_void_p, C.c_int]
         L.mko_integrate_history.argtypes = [_dp, C.c_long, C.c_int, C.c_double, C.c_double, C.c_double,
                                             C.c_long, _dp, _dp, C.c_int]
         L.mko_bisect_iterations.argtypes = [C.c_double, C.c_int]
@@ -50,7 +50,7 @@ def lib():
         L.mko_synchrotron.argtypes = [_dp] * 5 + [C.c_long, C.c_int, C.c_double, _dp, _dp]
         L.mko_render.argtypes = ([_fp, C.c_long, C.c_int, _dp, _dp, _dp, _dp, _ip, C.c_int, _dp, C.c_double]
                                  + [C.c_double] * 4 + [C.c_int, C.c_int] + [C.c_double] * 5
-                                 + [C.c_long, C.c_long, C.c_long, _dp, _ip, C.c_int])
+                                 + [C.c_long, C.c_long, C.c_long, _dp, _ip, C.c_void_p, C.c_void_p, C.c_int])
         L.mko_max_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -96,14 +96,19 @@ def camera_pixels(a, inc, dist, radius, angle):
     return out
 
 
-def integrate_final(s0, N, div, tol, a, nthreads=None):
+def integrate_final(s0, N, div, tol, a, nthreads=None, margins=False):
+    """-> (K, last state, select value) per ray, and with margins=True the
+    (n, 2) threshold margins of the rejected candidate and the last accepted
+    state (see CSnapshot.render)."""
     s0 = np.ascontiguousarray(s0, dtype=np.float64)
     n = s0.shape[0]
     K = np.empty(n, dtype=np.int32)
     fin = np.empty((n, 8))
     sel = np.empty(n)
-    lib().mko_integrate_final(s0, n, N, div, tol, a, K, fin, sel, threads(nthreads))
-    return K, fin, sel
+    mg = np.empty((n, 2)) if margins else None
+    lib().mko_integrate_final(s0, n, N, div, tol, a, K, fin, sel,
+                              None if mg is None else mg.ctypes.data_as(C.c_void_p), threads(nthreads))
+    return (K, fin, sel, mg) if margins else (K, fin, sel)
 
 
 def trim_rows(K, N):
@@ -169,16 +174,30 @@ class CSnapshot:
         return (out, mb.reshape(S.shape[:-1])) if with_blocks else out
 
     def render(self, a, inc, dist, fov, res, max_nsteps, fluid_gamma, r_high, mass_scale, M_bh, nu_obs,
-               p0=0, count=None, stride=1, nthreads=None):
-        """Pixels p0 + q * stride (q < count) of the res x res image."""
+               p0=0, count=None, stride=1, nthreads=None, force_steps=None, margins=False):
+        """Pixels p0 + q * stride (q < count) of the res x res image -> (I, steps).
+
+        force_steps: accepted steps per ray to run instead of the freeze rule
+        (a candidate the rule rejects is accepted with the unclamped step).
+        margins: also return (count, 2) relative distances of the rejected
+        candidate and of the last accepted state from the nearest freeze
+        threshold (|r - r_EH| against tol or 1500; inf where not defined).
+        """
         count = (res * res - p0 + stride - 1) // stride if count is None else count
         I = np.empty(count)
         steps = np.empty(count, dtype=np.int32)
         grid = self.grid if self.grid is not None else np.full((1, 1, 1), -1, dtype=np.int32)
+        fs = None
+        if force_steps is not None:
+            fs = np.ascontiguousarray(force_steps, dtype=np.int32).reshape(-1)
+            assert fs.size == count
+        mg = np.empty((count, 2)) if margins else None
         lib().mko_render(self.data, self.nmb, self.n, self.lo, self.hi, self.v0, self.dx, grid, self.G,
                          self.g0, self.gh, a, inc, dist, fov, res, max_nsteps, fluid_gamma, r_high, mass_scale,
-                         M_bh, nu_obs, p0, count, stride, I, steps, threads(nthreads))
-        return I, steps
+                         M_bh, nu_obs, p0, count, stride, I, steps,
+                         None if fs is None else fs.ctypes.data_as(C.c_void_p),
+                         None if mg is None else mg.ctypes.data_as(C.c_void_p), threads(nthreads))
+        return (I, steps, mg) if margins else (I, steps)
 
 
 def synchrotron(Ne, Th, B, pitch, nu, invariant=True, rescale=1.0):

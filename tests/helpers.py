@@ -94,3 +94,43 @@ def small_two_level(n=4, seed=0, bhspin=0.5):
     ds = {k: geo[k] for k in ("x1v", "x2v", "x3v", "x1f", "x2f", "x3f", "Levels", "LogicalLocations")}
     ds.update(uov=fields[:5], B=fields[5:], VariableNames=list(syn.VARIABLE_NAMES))
     return ds
+
+
+MSUN = 1.989e33
+
+
+def image_parity_vs_c(img, steps, cs, a, inc, res, nsteps=10000, nu=230e9, fov=20.0, rtol=1e-4, flux_rtol=1e-5,
+                      label=""):
+    """Plain FP64 GPU image and step counts against the C restatement of
+    make_image (images.py:30-144): the same accepted-step count on every ray,
+    the same finite/NaN pattern, every pixel with |I| above 1e-6 of the peak
+    within rtol, and the total flux within flux_rtol (of the summed |I|, so
+    that images with negative pixels -- the explicit-Euler transfer of
+    transfer.py:106-119 is unstable where alpha |dt| L > 2 -- are held to the
+    same bar).  Prints the counts.  A step-count mismatch is reported with the
+    relative distance of the deciding candidate from the freeze threshold
+    (geodesics.py:249-252), which is rounding-small when that is its cause."""
+    ref, rst, mg = cs.render(a, inc, 1000.0, fov, res, nsteps, 13. / 9, 40.0, 1e26, 6.2e9 * MSUN, nu, margins=True)
+    img = np.asarray(img).ravel()
+    steps = np.asarray(steps).ravel()
+    fin = np.isfinite(ref)
+    peak = float(np.max(np.abs(ref[fin]))) if fin.any() else 0.0
+    mask = fin & (np.abs(ref) > 1e-6 * peak)
+    with np.errstate(all="ignore"):
+        rel = np.abs(img[mask] - ref[mask]) / np.abs(ref[mask])
+    nbad = int(np.count_nonzero(~(rel <= rtol)))
+    dK = steps - rst
+    mism = np.flatnonzero(dK)
+    total = float(np.sum(np.abs(ref[fin])))
+    flux = abs(float(np.sum(img[fin])) - float(np.sum(ref[fin]))) / total if total > 0 else 0.0
+    print(f"[parity] {label} {res}^2: {int(mask.sum())} pixels checked, {nbad} beyond {rtol:g} "
+          f"(max rel {float(rel.max()) if rel.size else 0.0:.2e}), step counts differ on {mism.size} of {steps.size} "
+          f"rays, flux rel err {flux:.2e}, negative pixels {int(np.count_nonzero(ref[fin] < 0))}")
+    detail = [(int(q), int(steps[q]), int(rst[q]), float(mg[q, 0] if dK[q] > 0 else mg[q, 1])) for q in mism[:10]]
+    assert mism.size == 0, f"step counts differ: (ray, K gpu, K c, threshold margin) {detail}"
+    np.testing.assert_array_equal(np.isfinite(img), fin)
+    assert nbad == 0, f"{nbad} pixels beyond {rtol:g}"
+    assert flux <= flux_rtol, f"flux rel err {flux:.3e}"
+    if not fin.any() or peak == 0.0:
+        assert np.all(img[fin] == 0.0)
+    return ref.reshape(res, res)

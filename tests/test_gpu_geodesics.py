@@ -80,12 +80,19 @@ def test_integrate_final_64x64_kerr_camera():
     _lib.check(_lib.lib().mk_integrate_final(d0.ptr, n, 10000, 40.0, 0.01 * rEH, a, dK.ptr, dl.ptr, ds.ptr, None))
     K = dK.to_array((n,), np.int32)
     last = dl.to_array((n, 8))
-    Kc, lastc, _ = c_oracle.integrate_final(s0, 10000, 40.0, 0.01 * rEH, a)
-    assert np.mean(K == Kc) >= 0.999
-    same = K == Kc
-    r_g = onp.radius_cal(last[same], a)
-    r_c = onp.radius_cal(lastc[same], a)
-    np.testing.assert_allclose(r_g, r_c, rtol=1e-6)
+    Kc, lastc, selc, mg = c_oracle.integrate_final(s0, 10000, 40.0, 0.01 * rEH, a, margins=True)
+    mism = np.flatnonzero(K != Kc)
+    r_g = onp.radius_cal(last, a)
+    r_c = onp.radius_cal(lastc, a)
+    sel = ds.to_array((n,))
+    print(f"[parity] config 1 64^2: frozen row differs on {mism.size} of {n} rays; termination radius max rel "
+          f"{np.max(np.abs(r_g - r_c) / r_c):.2e}; select value max rel {np.max(np.abs(sel - selc) / np.abs(selc)):.2e}")
+    detail = [(int(q), int(K[q]), int(Kc[q]), float(mg[q, 0] if K[q] > Kc[q] else mg[q, 1])) for q in mism[:10]]
+    assert mism.size == 0, f"frozen rows differ: (ray, K gpu, K c, threshold margin) {detail}"
+    # termination radii (north_star: 1e-6 M) and the photon-selection value
+    assert np.max(np.abs(r_g - r_c)) < 1e-6
+    np.testing.assert_allclose(r_g, r_c, rtol=1e-9)
+    np.testing.assert_allclose(sel, selc, rtol=1e-9)
     T = C.c_long()
     _lib.check(_lib.lib().mk_trim_rows(dK.ptr, n, 10000, C.byref(T), None))
     assert T.value == c_oracle.trim_rows(Kc, 10000)

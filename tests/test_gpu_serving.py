@@ -111,9 +111,8 @@ def test_budget_back_pressure_and_errors(models):
         st = srv.stats()
         assert st["hbm_budget"] == need and st["hbm_in_use"] <= need
         outs = [srv.result(r, timeout=120) for r in rids]
-        for o in outs:
-            np.testing.assert_allclose(o, ref, rtol=1e-4, atol=1e-6 * ref.max())
-        np.testing.assert_array_equal(o, outs[0])
+        for o in outs:  # served = mk_render, bit for bit, under back-pressure too
+            np.testing.assert_array_equal(o, ref)
         with pytest.raises(_lib.MkError, match="budget"):
             srv.submit_image(sid, **_img_kw(64, 0.5, 30.0, 230e9))
         with pytest.raises(_lib.MkError, match="unknown snapshot"):
