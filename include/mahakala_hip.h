@@ -107,6 +107,18 @@ int mk_select_photons(double inclination_deg, const double* h_angle, const doubl
 int mk_shadow_bisect(double bhspin, double inclination_deg, const double* h_angles, long n, int max_steps,
                      double error_allowed, int max_it, double distance, double* h_radii);
 
+/* mk_shadow_bisect without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (request
+ * loops that must not call hipMalloc, harnesses that forbid it).  The same
+ * arguments and the same radii, bit for bit, as mk_shadow_bisect; the
+ * staging of angles and radii lives in a fixed library-owned array of
+ * 3 * MK_NOALLOC_MAX_ANGLES doubles (1.5 MiB) per GPU, held until the process
+ * ends.  At most MK_NOALLOC_MAX_ANGLES angles (MK_ERR_ARG beyond).  Calls are
+ * serialised by a library lock, so they never share the array at once. */
+#define MK_NOALLOC_MAX_ANGLES 65536
+int mk_shadow_bisect_noalloc(double bhspin, double inclination_deg, const double* h_angles, long n, int max_steps,
+                             double error_allowed, int max_it, double distance, double* h_radii);
+
 /* ---------------------------------------------------------------- transfer */
 /* synchrotron_coefficients  (transfer.py:30-86), elementwise over n. */
 int mk_synchrotron(const double* d_Ne, const double* d_Theta_e, const double* d_B, const double* d_pitch,

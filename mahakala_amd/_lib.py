@@ -103,6 +103,7 @@ _SIGS = {
     "mk_integrate_history": [vp, lng, C.c_int, dbl, dbl, dbl, lng, vp, vp, vp],
     "mk_select_photons": [dbl, vp, vp, lng, dbl, dbl, C.c_int, vp],
     "mk_shadow_bisect": [dbl, dbl, vp, lng, C.c_int, dbl, C.c_int, dbl, vp],
+    "mk_shadow_bisect_noalloc": [dbl, dbl, vp, lng, C.c_int, dbl, C.c_int, dbl, vp],
     "mk_synchrotron": [vp, vp, vp, vp, vp, lng, C.c_int, dbl, vp, vp, vp],
     "mk_solve_intensity": [vp, vp, vp, lng, lng, dbl, vp, vp, vp],
     "mk_attenuated_emissivity": [vp, vp, vp, lng, lng, dbl, vp, vp],

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "mk_device.h"
@@ -357,6 +358,38 @@ double horizon(double a) { return 1.0 + std::sqrt(1.0 - a * a); }
 
 using namespace mk;
 
+// ------------------------------------------------ allocation-free staging
+// mk_shadow_bisect_noalloc keeps its staging (cos and sin of the angles in,
+// radii out) in this fixed array instead of allocating: each GPU holds its
+// own copy (1.5 MiB) until the process ends.  Host code reaches it only
+// through the per-device address from hipGetSymbolAddress; the array's name
+// in host code is not a device address.  The host side of the staging is a
+// fixed host array too.  One lock serialises the calls, so calls from
+// several host threads do not share the array at once.
+__device__ __align__(256) double scratch[3 * MK_NOALLOC_MAX_ANGLES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_dev[kMaxDevices] = {};
+double g_noalloc_host[2 * MK_NOALLOC_MAX_ANGLES];
+
+// The current device's copy of `scratch`, looked up once per device (called
+// with g_noalloc_mu held).
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  MK_HIP(hipGetDevice(&dev));
+  MK_REQUIRE(dev >= 0 && dev < kMaxDevices, "mk_shadow_bisect_noalloc: device %d beyond %d", dev, kMaxDevices);
+  if (!g_noalloc_dev[dev]) {
+    void* ptr = nullptr;
+    MK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_noalloc_dev[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_noalloc_dev[dev];
+  return MK_OK;
+}
+}  // namespace
+
 // =================================================================== C ABI
 extern "C" {
 
@@ -621,21 +654,57 @@ int mk_select_photons(double inc, const double* h_angle, const double* h_radius,
   return rc;
 }
 
+// Halvings of a bisection with the reference's shared bracket
+// (geodesics.py:417-431): every angle does the same number.
+static int bisect_iterations(double error_allowed, int max_it) {
+  int iters = 0;
+  double inner = 0.5, outer = 10.0;
+  while (outer - inner > error_allowed && iters < max_it) {
+    inner = (outer - inner) / 2 + inner;
+    ++iters;
+  }
+  return iters;
+}
+
+// Launches the bisection of n angles (cos and sin in dc, ds; device) into dr
+// on the null stream.
+static hipError_t launch_bisect(double a, double inc, const double* dc, const double* ds, long n, int max_steps,
+                                int iters, double dist, double* dr) {
+  double ci, si;
+  cam_angles(inc, &ci, &si);
+  // K halvings per round with 2^K lanes per angle: the fewest rounds with at
+  // most about 65 536 lanes at once, then the halvings spread evenly over
+  // those rounds (fewer lanes for the same number of rounds; the slowest
+  // integration of a round sets its time).  K = 1: one lane per angle.
+  int Kmax = 1;
+  while (Kmax < 10 && Kmax < iters && n * (2L << Kmax) <= 65536) ++Kmax;
+  const int rounds = (iters + Kmax - 1) / Kmax;
+  const int K = rounds > 0 ? (iters + rounds - 1) / rounds : 1;
+  if (K == 1 || iters <= 1) {
+    k_bisect<<<grid_for(n, 64), 64>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, horizon(a), dr);
+  } else {
+    // one angle per block (2^K lanes, two or more waves) while the blocks
+    // still fit one per CU: fewer waves per CU run the FP64 walk faster
+    // (64 angles 8.9 -> 7.0 ms, 256 angles 8.6 -> 7.6 ms); beyond that,
+    // blocks of at least 256 lanes (512 angles: 8.4 against 9.6 ms)
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 0;
+    const bool one = (1 << K) >= 64 && n <= cus;
+    const int block = one ? (1 << K) : ((1 << K) >= 256 ? (1 << K) : 256), G = block >> K;
+    k_bisect_tree<<<(unsigned)((n + G - 1) / G), block>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, horizon(a),
+                                                           K, dr);
+  }
+  return hipGetLastError();
+}
+
 int mk_shadow_bisect(double a, double inc, const double* h_angles, long n, int max_steps, double error_allowed,
                      int max_it, double dist, double* h_radii) {
   MK_REQUIRE(n >= 0 && max_steps >= 1 && h_angles && h_radii, "mk_shadow_bisect: bad arguments");
   if (!n) return MK_OK;
   // every angle shares one bracket width, so the count is fixed up front
-  int iters = 0;
-  {
-    double inner = 0.5, outer = 10.0;
-    while (outer - inner > error_allowed && iters < max_it) {
-      inner = (outer - inner) / 2 + inner;
-      ++iters;
-    }
-  }
-  double ci, si;
-  cam_angles(inc, &ci, &si);
+  const int iters = bisect_iterations(error_allowed, max_it);
   std::vector<double> hc(n), hs(n);
   for (long p = 0; p < n; ++p) {
     hc[p] = std::cos(h_angles[p]);
@@ -647,39 +716,46 @@ int mk_shadow_bisect(double a, double inc, const double* h_angles, long n, int m
   if (e == hipSuccess) e = hipMalloc(&dr, n * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(dc, hc.data(), n * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(ds, hs.data(), n * sizeof(double), hipMemcpyHostToDevice);
-  if (e == hipSuccess) {
-    // K halvings per round with 2^K lanes per angle: the fewest rounds with at
-    // most about 65 536 lanes at once, then the halvings spread evenly over
-    // those rounds (fewer lanes for the same number of rounds; the slowest
-    // integration of a round sets its time).  K = 1: one lane per angle.
-    int Kmax = 1;
-    while (Kmax < 10 && Kmax < iters && n * (2L << Kmax) <= 65536) ++Kmax;
-    const int rounds = (iters + Kmax - 1) / Kmax;
-    const int K = rounds > 0 ? (iters + rounds - 1) / rounds : 1;
-    if (K == 1 || iters <= 1) {
-      k_bisect<<<grid_for(n, 64), 64>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, horizon(a), dr);
-    } else {
-      // one angle per block (2^K lanes, two or more waves) while the blocks
-      // still fit one per CU: fewer waves per CU run the FP64 walk faster
-      // (64 angles 8.9 -> 7.0 ms, 256 angles 8.6 -> 7.6 ms); beyond that,
-      // blocks of at least 256 lanes (512 angles: 8.4 against 9.6 ms)
-      int dev = 0, cus = 0;
-      if (hipGetDevice(&dev) != hipSuccess ||
-          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        cus = 0;
-      const bool one = (1 << K) >= 64 && n <= cus;
-      const int block = one ? (1 << K) : ((1 << K) >= 256 ? (1 << K) : 256), G = block >> K;
-      k_bisect_tree<<<(unsigned)((n + G - 1) / G), block>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, horizon(a),
-                                                             K, dr);
-    }
-    e = hipGetLastError();
-  }
+  if (e == hipSuccess) e = launch_bisect(a, inc, dc, ds, n, max_steps, iters, dist, dr);
   if (e == hipSuccess) e = hipMemcpy(h_radii, dr, n * sizeof(double), hipMemcpyDeviceToHost);
   (void)hipFree(dc);
   (void)hipFree(ds);
   (void)hipFree(dr);
   if (e != hipSuccess) {
     set_error("mk_shadow_bisect: %s", hipGetErrorString(e));
+    return MK_ERR_HIP;
+  }
+  return MK_OK;
+}
+
+int mk_shadow_bisect_noalloc(double a, double inc, const double* h_angles, long n, int max_steps,
+                             double error_allowed, int max_it, double dist, double* h_radii) {
+  MK_REQUIRE(n >= 0 && max_steps >= 1 && h_angles && h_radii, "mk_shadow_bisect_noalloc: bad arguments");
+  MK_REQUIRE(n <= MK_NOALLOC_MAX_ANGLES, "mk_shadow_bisect_noalloc: %ld angles exceed the limit of %d", n,
+             (int)MK_NOALLOC_MAX_ANGLES);
+  if (!n) return MK_OK;
+  const int iters = bisect_iterations(error_allowed, max_it);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  double* scr = nullptr;
+  int rc = noalloc_scratch(&scr);
+  if (rc) return rc;
+  double* hc = g_noalloc_host;
+  double* hs = g_noalloc_host + MK_NOALLOC_MAX_ANGLES;
+  for (long p = 0; p < n; ++p) {
+    hc[p] = std::cos(h_angles[p]);
+    hs[p] = std::sin(h_angles[p]);
+  }
+  // scratch: cos[0, MAX), sin[MAX, 2 MAX), radii[2 MAX, 3 MAX); every value
+  // a launch reads (p < n) is written first, so nothing needs clearing
+  double* dc = scr;
+  double* ds = scr + MK_NOALLOC_MAX_ANGLES;
+  double* dr = scr + 2 * MK_NOALLOC_MAX_ANGLES;
+  hipError_t e = hipMemcpy(dc, hc, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(ds, hs, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_bisect(a, inc, dc, ds, n, max_steps, iters, dist, dr);
+  if (e == hipSuccess) e = hipMemcpy(h_radii, dr, n * sizeof(double), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_error("mk_shadow_bisect_noalloc: %s", hipGetErrorString(e));
     return MK_ERR_HIP;
   }
   return MK_OK;
