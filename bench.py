@@ -223,6 +223,110 @@ def heavy_stream(models, seed=1, n_images=128, n_shadow=128, res=512, n_angles=4
     return out
 
 
+def stage_stream(L, res=256, reps=3):
+    """The unfused path's standalone kernels on the config-2 sample stream:
+    the materialised history of the 256^2 image (T rows of every ray, as
+    geodesic_integrator returns it), the sampler k_sample_scalars over all
+    T x npx samples (stage b, athenak.py:639-812), then k_synchrotron over the
+    same samples and k_solve over the (T, npx) arrays (stage c,
+    transfer.py:30-119), each launched `reps` times and timed with HIP events.
+    Reports each kernel's time and its algorithmic bytes per second against
+    8 TB/s: sampler 136 B per sample (state 64 B in, one 32-byte cell record,
+    5 scalars 40 B out), synchrotron 56 B (5 inputs, j and alpha), solve 24 B
+    per sample (j, alpha, dt) + 8 B per ray."""
+    import ctypes as C
+    from mahakala_amd.grmhd.athenak import AthenakFluidModel
+    from mahakala_amd.grmhd.synthetic import uniform_torus_snapshot
+    a, inc, nu, fov = 0.5, 30.0, 230e9, 20.0
+    model = AthenakFluidModel.from_datasets(uniform_torus_snapshot(a, seed=0), a, fluid_gamma=13. / 9)
+    lib = L.lib()
+    n = res * res
+    d0 = L.DeviceBuffer(8 * 8 * n)
+    L.check(lib.mk_camera_grid(a, inc, 1000.0, -fov / 2, fov / 2, res, d0.ptr, None))
+    dK = L.DeviceBuffer(4 * n)
+    L.check(lib.mk_integrate_final(d0.ptr, n, 10000, 40.0, 1e-4, a, dK.ptr, None, None, None))
+    T = C.c_long()
+    L.check(lib.mk_trim_rows(dK.ptr, n, 10000, C.byref(T), None))
+    T = T.value
+    ns = T * n
+    dS, dD = L.DeviceBuffer(8 * 8 * ns), L.DeviceBuffer(8 * ns)
+    L.check(lib.mk_integrate_history(d0.ptr, n, 10000, 40.0, 1e-4, a, T, dS.ptr, dD.ptr, None))
+    dsc = L.DeviceBuffer(5 * 8 * ns)
+
+    def sampler():
+        L.check(lib.mk_sample_scalars(model.device_snapshot.handle, dS.ptr, ns, a, np.pi / 3, dsc.ptr, None))
+    ms_b, _ = time_launches(L, sampler, reps, 1)
+    # the per-sample glue of make_image (images.py:86-118) on the host, as the
+    # unfused path does, to give the emission kernel its real inputs
+    sc = dsc.to_array((5, ns))
+    del dS
+    units = model.get_units(6.2e9 * MSUN, 1e26)
+    with np.errstate(all="ignore"):
+        bsq = sc[4] * sc[4]
+        beta = sc[1] * (13. / 9 - 1.) / bsq / 0.5
+        tr = (40.0 * beta * beta + 1.0) / (1.0 + beta * beta)
+        theta = (2.99792458e10 ** 2 * (1.6726e-24 * sc[1] * (1 / 3.) * (2 / 3.))
+                 / (sc[0] * (2 / 3. + (1 / 3.) * tr)) / (9.1094e-28 * 2.99792458e10 ** 2))
+        ins = [units["Ne_unit"] * sc[0], theta, units["B_unit"] * sc[4], sc[2], -sc[3] * nu]
+    del sc, bsq, beta, tr, theta
+    dins = [L.DeviceBuffer.from_array(v) for v in ins]
+    del ins
+    dj, dal = L.DeviceBuffer(8 * ns), L.DeviceBuffer(8 * ns)
+    dI = L.DeviceBuffer(8 * n)
+
+    def synch():
+        L.check(lib.mk_synchrotron(*[b.ptr for b in dins], ns, 1, 1.0 / nu, dj.ptr, dal.ptr, None))
+
+    def solve():
+        L.check(lib.mk_solve_intensity(dj.ptr, dal.ptr, dD.ptr, T, n, units["L_unit"], dI.ptr, None, None))
+    ms_s, _ = time_launches(L, synch, reps, 1)
+    ms_t, _ = time_launches(L, solve, reps, 1)
+    out = {"workload": f"materialised history of the {res}^2 config-2 image (a=0.5, i=30 deg, 230 GHz, torus on "
+                       f"512 blocks of 32^3): T = {T} rows x {n} rays = {ns} samples",
+           "samples": int(ns), "T": int(T)}
+    for name, ms, bps, extra in (("k_sample_scalars", ms_b, 136, 0), ("k_synchrotron", ms_s, 56, 0),
+                                 ("k_solve", ms_t, 24, 8 * n)):
+        t = float(np.mean(ms))
+        gbs = (bps * ns + extra) / (t * 1e-3) / 1e9
+        out[name] = {"ms": round(t, 4), "algorithmic_bytes": int(bps * ns + extra),
+                     "algorithmic_GBps": round(gbs, 1), "frac_of_8TBps": round(gbs / HBM_PEAK, 4)}
+        print(f"[stages] {name}: {t:.3f} ms, {gbs:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
+    return out
+
+
+def step_cycles(L, steps=2000, repeats=3):
+    """One wave alone on a SIMD repeats each stage of the per-ray loop
+    (mk_step_cycles): shader cycles per repetition, FP64 against mixed, and
+    the mixed-over-FP64 ratios they allow -- for the chain of one ray (RK4 +
+    radius + step rule: what bounds a launch whose longest ray sets its time)
+    and for a whole accepted step (geodesic + interpolation + emission: what
+    bounds a launch that keeps every SIMD busy)."""
+    names = ["rk4", "rk4", "geodesic_step", "geodesic_step", "interp", "interp", "interp_emission",
+             "interp_emission"]
+    out = {"fp64": {}, "mixed": {}}
+    ghz = []
+    for v in range(8):
+        best = None
+        for _ in range(repeats):
+            h = np.zeros(3)
+            L.check(L.lib().mk_step_cycles(v, steps, 0.5, h.ctypes.data))
+            best = h if best is None or h[0] < best[0] else best
+        out["fp64" if v % 2 == 0 else "mixed"][names[v]] = round(float(best[0]), 1)
+        ghz.append(float(best[2]))
+    f, m = out["fp64"], out["mixed"]
+    whole64 = f["geodesic_step"] + f["interp_emission"]
+    wholem = m["geodesic_step"] + m["interp_emission"]
+    out["what"] = ("shader cycles per repetition of one stage by one wave alone on a SIMD (2000 repetitions, best "
+                   "of 3); interp_emission includes the interpolation")
+    out["clock_ghz"] = round(float(np.median(ghz)), 3)
+    out["ratio_fp64_over_mixed"] = {
+        "rk4": round(f["rk4"] / m["rk4"], 3),
+        "geodesic_chain": round(f["geodesic_step"] / m["geodesic_step"], 3),
+        "interp_emission": round(f["interp_emission"] / m["interp_emission"], 3),
+        "whole_step": round(whole64 / wholem, 3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,6 +345,10 @@ def main():
     ap.add_argument("--no-shadow", action="store_true", help="skip the shadow bisection timings")
     ap.add_argument("--form", choices=["auto", "classic", "paired", "triple"], default="auto",
                     help="render kernel form (auto: the library's choice)")
+    ap.add_argument("--stages", action="store_true",
+                    help="only time the unfused sampler, emission and transfer kernels on the config-2 sample stream")
+    ap.add_argument("--step-cycles", action="store_true",
+                    help="only time the per-step stages in one wave (FP64 against mixed) and print them")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the device copy kernel over MIB MiB 3 times (traffic counter calibration)")
     args = ap.parse_args()
@@ -259,6 +367,12 @@ def main():
             L.check(L.lib().mk_device_copy(dst.ptr, src.ptr, nb, None))
         L.sync()
         print(json.dumps({"copy_calibration_bytes": nb, "launches": 3}))
+        return
+    if args.step_cycles:
+        print(json.dumps(step_cycles(L)))
+        return
+    if args.stages:
+        print(json.dumps(stage_stream(L)))
         return
     from mahakala_amd.grmhd.athenak import AthenakFluidModel
     from mahakala_amd.grmhd.synthetic import uniform_torus_snapshot
@@ -481,6 +595,25 @@ def main():
                                "sample": f"the full {res}x{res} image once, C FP64 restatement with OpenMP "
                                          f"({dt_cpu:.1f} s wall)",
                                "flux_rel_err_vs_gpu_plain": abs(float(I_cpu.sum()) - flux_p) / flux_p}
+        # the reference's own dataflow: the NumPy FP64 restatement, a scan over
+        # steps vectorised over rays (jacfwd by dual numbers, LU inverse), the
+        # materialised history and the host meshblock-mask sampler, on one core
+        from oracle import mahakala_np as onp
+        snap_np = onp.Snapshot(model.all_meshblocks, model.x1v, model.x2v, model.x3v, model.x1f, model.x2f,
+                               model.x3f, list(model.variable_names), a, 13. / 9)
+        s0 = onp.initialize_geodesics_at_camera(a, inc, 1000, -fov / 2., fov / 2., res)
+        idx = np.sort(np.random.Generator(np.random.PCG64(0)).choice(n, 256, replace=False))
+        t1 = time.perf_counter()
+        S_np, dt_np = onp.geodesic_integrator(10000, s0[idx], 40, 1e-4, a)
+        I_np = onp.image_from_history(snap_np, S_np, dt_np, 13. / 9, 1e26, 6.2e9 * MSUN, 40.0, nu)
+        dt_np_s = time.perf_counter() - t1
+        ref = I_plain.ravel()[idx]
+        out["cpu_baseline_numpy"] = {
+            "value": round(idx.size / dt_np_s, 2), "unit": "rays/s", "cores": 1, "kind": "port",
+            "sample": f"{idx.size} rays drawn at random (seed 0) from the {res}x{res} image, NumPy FP64 "
+                      f"restatement with the reference's dataflow, T = {S_np.shape[0]} rows ({dt_np_s:.1f} s)",
+            "pixel_max_rel_err_vs_gpu_plain": float(np.max(np.abs(I_np - ref)[ref > 0] / ref[ref > 0]))}
+        print(f"[bench] NumPy restatement: {idx.size / dt_np_s:.1f} rays/s", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -41,6 +41,8 @@ const char* mk_last_error(void);
 int mk_version(void);
 /* device name, compute units and HBM size of the current device */
 int mk_device_info(char* h_name, size_t name_len, int* h_cus, size_t* h_total_bytes);
+/* free and total device memory of the current device (hipMemGetInfo) */
+int mk_device_mem(size_t* h_free_bytes, size_t* h_total_bytes);
 int mk_malloc(void** d_ptr, size_t bytes);
 int mk_free(void* d_ptr);
 int mk_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes, void* stream);
@@ -49,6 +51,12 @@ int mk_memset(void* d_dst, int value, size_t bytes, void* stream);
 /* device-to-device copy by a shader kernel (16-byte granules); no reference
  * counterpart: used to calibrate HBM traffic counters against known bytes */
 int mk_device_copy(void* d_dst, const void* d_src, size_t bytes, void* stream);
+/* one wave alone on a SIMD repeats one stage of the per-ray loop `steps`
+ * times -> h_out[3] = shader cycles and ns per repetition, and the clock
+ * (GHz); no reference counterpart: a measurement aid.  variant: 0/1 RK4
+ * step FP64/mixed, 2/3 RK4 + radius + step rule FP64/mixed, 4/5 trilinear
+ * interpolation FP64/FP32, 6/7 interpolation + emission FP64/FP32. */
+int mk_step_cycles(int variant, int steps, double bhspin, double* h_out);
 int mk_stream_create(void** stream);
 int mk_stream_destroy(void* stream);
 int mk_stream_sync(void* stream);

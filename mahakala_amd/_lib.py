@@ -76,12 +76,14 @@ _SIGS = {
     "mk_last_error": [],
     "mk_version": [],
     "mk_device_info": [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_size_t)],
+    "mk_device_mem": [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
     "mk_malloc": [C.POINTER(vp), C.c_size_t],
     "mk_free": [vp],
     "mk_memcpy_h2d": [vp, vp, C.c_size_t, vp],
     "mk_memcpy_d2h": [vp, vp, C.c_size_t, vp],
     "mk_memset": [vp, C.c_int, C.c_size_t, vp],
     "mk_device_copy": [vp, vp, C.c_size_t, vp],
+    "mk_step_cycles": [C.c_int, C.c_int, dbl, vp],
     "mk_stream_create": [C.POINTER(vp)],
     "mk_stream_destroy": [vp],
     "mk_stream_sync": [vp],
@@ -187,6 +189,13 @@ class DeviceBuffer:
         if getattr(self, "ptr", None) and _lib is not None:
             _lib.mk_free(self.ptr)
             self.ptr = None
+
+
+def device_memory():
+    """(free, total) bytes of the current device."""
+    free, total = C.c_size_t(), C.c_size_t()
+    check(lib().mk_device_mem(C.byref(free), C.byref(total)))
+    return free.value, total.value
 
 
 def device_info():

@@ -410,6 +410,12 @@ int mk_device_info(char* name, size_t len, int* cus, size_t* total) {
   return MK_OK;
 }
 
+int mk_device_mem(size_t* free_bytes, size_t* total_bytes) {
+  MK_REQUIRE(free_bytes && total_bytes, "mk_device_mem: null argument");
+  MK_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return MK_OK;
+}
+
 int mk_malloc(void** p, size_t bytes) {
   MK_REQUIRE(p != nullptr, "mk_malloc: null out pointer");
   *p = nullptr;

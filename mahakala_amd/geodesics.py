@@ -146,14 +146,36 @@ def initial_condition(s0_x, s0_v, bhspin):
     return wrap(out.to_array((n, 8)))
 
 
-def geodesic_integrator(N, s0, div, tol, bhspin):
+# Bytes of one materialised history row of one ray: state (8) and dt, FP64.
+HISTORY_ROW_BYTES = 9 * 8
+# Share of the free device memory a history chunk may take when no explicit
+# budget is given.
+DEVICE_BUDGET_FRACTION = 0.5
+
+
+def history_chunk_rays(n, T, max_chunk_bytes=None):
+    """Rays per chunk of a materialised (T, n) history: as many as fit
+    max_chunk_bytes of device memory (T rows of 72 bytes per ray), or, when it
+    is None, half of the free device memory; at least one."""
+    budget = max_chunk_bytes
+    if budget is None:
+        free, _ = _lib.device_memory()
+        budget = int(free * DEVICE_BUDGET_FRACTION)
+    per_ray = max(1, int(T)) * HISTORY_ROW_BYTES
+    return int(max(1, min(n, budget // per_ray)))
+
+
+def geodesic_integrator(N, s0, div, tol, bhspin, max_chunk_bytes=None):
     """RK4 geodesics with the radius-scaled step  (geodesics.py:233-281).
 
     Returns (S (T, npx, 8), final_dt (T, npx)) with the reference's fixed-N
     scan semantics: a ray is frozen at its first rejected step and repeats
     (state, 0) afterwards; T = argmax(all frozen) + 2, or N if that is < 1.
-    The GPU integrates each ray only until it freezes (one pass to find T,
-    one to write the T rows).
+    The GPU integrates each ray only until it freezes: one pass over all rays
+    finds T, then the T rows are written in chunks of rays whose history fits
+    max_chunk_bytes of device memory (None: half of the free device memory),
+    so a history larger than the GPU's memory is still returned whole.
+    Chunking changes no value: every ray is integrated alone.
     """
     s0 = np.ascontiguousarray(np.asarray(s0, dtype=np.float64))
     n = s0.shape[0]
@@ -166,11 +188,23 @@ def geodesic_integrator(N, s0, div, tol, bhspin):
     T = C.c_long()
     _lib.check(L.mk_trim_rows(d_K.ptr, n, N, C.byref(T), None))
     T = T.value
-    d_S = _lib.DeviceBuffer(8 * 8 * T * n)
-    d_D = _lib.DeviceBuffer(8 * T * n)
-    _lib.check(L.mk_integrate_history(d_s0.ptr, n, N, float(div), float(tol), float(bhspin), T, d_S.ptr, d_D.ptr,
-                                      None))
-    return wrap(d_S.to_array((T, n, 8))), wrap(d_D.to_array((T, n)))
+    del d_K
+    m = history_chunk_rays(n, T, max_chunk_bytes) if n else 1
+    S = np.empty((T, n, 8))
+    D = np.empty((T, n))
+    d_S = _lib.DeviceBuffer(8 * 8 * T * m)
+    d_D = _lib.DeviceBuffer(8 * T * m)
+    for lo in range(0, n, m):
+        k = min(m, n - lo)
+        _lib.check(L.mk_integrate_history(d_s0.ptr + 8 * 8 * lo, k, N, float(div), float(tol), float(bhspin), T,
+                                          d_S.ptr, d_D.ptr, None))
+        if k == n:
+            _lib.check(L.mk_memcpy_d2h(S.ctypes.data, d_S.ptr, S.nbytes, None))
+            _lib.check(L.mk_memcpy_d2h(D.ctypes.data, d_D.ptr, D.nbytes, None))
+        else:
+            S[:, lo:lo + k] = d_S.to_array((T, k, 8))
+            D[:, lo:lo + k] = d_D.to_array((T, k))
+    return wrap(S), wrap(D)
 
 
 def radius_cal(x, bhspin):

@@ -29,6 +29,10 @@ PRECISIONS = {"plain64": 0, "fp64": 0, "mixed": 1, "mixed-zone": 1}
 # relative and costs ~30% in time, so plain "mixed" leaves it off.
 ZONES = {"mixed": (0.0, 0.0, 0.0), "mixed-zone": (4.0, 5.0, 0.1)}
 
+# Device bytes per (row, ray) of the unfused pipeline at its peak, the
+# sampler: the states in (8 FP64) and five scalars out, rounded up.
+UNFUSED_ROW_BYTES = 128
+
 
 def render_params(bhspin, camera_inclination=60, camera_distance=1000, mass_scale=1.e26, M_bh=6.2e9 * Msun,
                   r_high=40, observing_frequency=230.e9, fov=20, resolution=160, max_nsteps=10000,
@@ -89,14 +93,19 @@ def make_image(fluid_model, camera_inclination=60, camera_distance=1000, mass_sc
     bhspin = fluid_model.bhspin
     fluid_gamma = fluid_model.fluid_gamma
     npx = int(resolution) ** 2
+    snap = getattr(fluid_model, "device_snapshot", None)
     num_pixels_per_chunk = npx + 10
     if max_chunk_bytes is not None:
-        num_pixels_per_chunk = int(max_chunk_bytes // 4 // 20 // max_nsteps)
+        num_pixels_per_chunk = max(1, int(max_chunk_bytes // 4 // 20 // max_nsteps))
+    elif snap is None:
+        # no budget given: chunks whose per-step arrays fit half of the free
+        # device memory at the longest possible history (max_nsteps rows)
+        free, _ = _lib.device_memory()
+        num_pixels_per_chunk = max(1, min(npx + 10, int(free // 2 // UNFUSED_ROW_BYTES // max_nsteps)))
     print(num_pixels_per_chunk, npx)
     if fluid_gamma is None:
         raise TypeError("fluid_model.fluid_gamma must be set to compute beta (images.py:88)")
 
-    snap = getattr(fluid_model, "device_snapshot", None)
     if snap is not None:
         p = render_params(bhspin, camera_inclination, camera_distance, mass_scale, M_bh, r_high,
                           observing_frequency, fov, resolution, max_nsteps, fluid_gamma, precision)

@@ -138,3 +138,29 @@ def test_shadow_rounds_of_several_halvings_equal_the_sequential_bisection():
 def test_schwarzschild_shadow():
     r = ma.find_shadow_bisection_angles(0.0, 17.0, np.linspace(0, 6, 12))
     assert np.all(np.abs(r - np.sqrt(27)) < 9.5 / 2 ** 14 + 1e-3)
+
+
+def test_history_chunked_by_device_budget_equals_one_chunk():
+    """geodesic_integrator writes a history larger than its device budget in
+    chunks of rays instead of failing (the reference chunks make_image by
+    max_chunk_bytes, images.py:62-78; geodesics.py:233-281 returns one
+    array): the same S and final_dt bits as one chunk, with the global T."""
+    a = 0.9
+    s0 = np.asarray(ma.initialize_geodesics_at_camera(a, 60, 1000, -10, 10, 24))
+    S1, D1 = G.geodesic_integrator(10000, s0, 40, 1e-4, a, max_chunk_bytes=1 << 40)
+    T, n = S1.shape[:2]
+    budget = T * G.HISTORY_ROW_BYTES * 100   # 100 rays per chunk: 6 chunks, the last partial
+    assert G.history_chunk_rays(n, T, budget) == 100
+    S2, D2 = G.geodesic_integrator(10000, s0, 40, 1e-4, a, max_chunk_bytes=budget)
+    print(f"[planner] T = {T}, {n} rays, chunks of 100 rays: S and final_dt identical "
+          f"{np.array_equal(S1, S2) and np.array_equal(D1, D2)}")
+    np.testing.assert_array_equal(S2, S1)
+    np.testing.assert_array_equal(D2, D1)
+    S3, D3 = G.geodesic_integrator(10000, s0, 40, 1e-4, a)   # default: half of the free device memory
+    np.testing.assert_array_equal(S3, S1)
+    np.testing.assert_array_equal(D3, D1)
+    # one ray per chunk, and the trim rule still global
+    S4, D4 = G.geodesic_integrator(10000, s0[:7], 40, 1e-4, a, max_chunk_bytes=1)
+    S5, D5 = G.geodesic_integrator(10000, s0[:7], 40, 1e-4, a)
+    np.testing.assert_array_equal(S4, S5)
+    np.testing.assert_array_equal(D4, D5)

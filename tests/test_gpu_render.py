@@ -48,6 +48,12 @@ def test_image_fixture_from_numpy_oracle():
                             observing_frequency=float(f["nu"]))
     rel, _ = _pixel_close(gen, ref)
     assert rel.max() < 1e-4
+    # chunked by max_chunk_bytes (the reference's rule, images.py:62-69): 13
+    # pixels per chunk here, the same image bit for bit (frozen rows add 0)
+    chunked = images.make_image(_Unfused(m), camera_inclination=float(f["inc"]), fov=float(f["fov"]),
+                                resolution=int(f["res"]), mass_scale=float(f["mass_scale"]),
+                                observing_frequency=float(f["nu"]), max_chunk_bytes=13 * 4 * 20 * 10000)
+    np.testing.assert_array_equal(chunked, gen)
 
 
 class _Unfused:

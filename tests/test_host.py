@@ -65,3 +65,15 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_history_chunk_planner_arithmetic():
+    """Rays per chunk of a materialised history: T rows of 72 bytes per ray
+    within the budget, at least one ray, at most all of them."""
+    from mahakala_amd import geodesics as G
+    assert G.HISTORY_ROW_BYTES == 72
+    assert G.history_chunk_rays(576, 1000, 72 * 1000 * 100) == 100
+    assert G.history_chunk_rays(576, 1000, 72 * 1000 * 100 - 1) == 99
+    assert G.history_chunk_rays(576, 1000, 1) == 1
+    assert G.history_chunk_rays(576, 1000, 1 << 50) == 576
+    assert G.history_chunk_rays(576, 0, 72 * 10) == 10

@@ -133,3 +133,30 @@ def test_select_photons_reads_row_before_argmax():
     np.testing.assert_array_equal(onp.select_photons_integrator(inc, ang, rad, a), expect)
     _, _, sel = c_oracle.integrate_final(onp.initial_condition(sx, sv, a), 2000, 40, 1e-2, a)
     np.testing.assert_allclose(sel, expect, rtol=1e-8)
+
+
+@pytest.mark.parametrize("a", [0.0, 0.7])
+def test_equator_camera_closed_form(a):
+    """The 'equator' camera (geodesics.py:183-197): a line of rays at x = d,
+    y = the pixel centres linspace(lo, hi, 2n+1)[1::2] = lo + (2k+1)(hi-lo)/2n,
+    z = 0 whatever the inclination, all with k = (1, 1, 0, 0) before
+    normalisation; after nullify (geodesics.py:58-85) k^t = 1, k^y = k^z = 0,
+    k^x > 0 and g(k, k) = 0."""
+    n, lo, hi, d = 60, -15.0, 15.0, 1000.0
+    for inc in (0.0, 60.0):
+        x, v = onp.get_initial_grid(inc, d, lo, hi, n, "equator")
+        yk = lo + (2 * np.arange(n) + 1) * (hi - lo) / (2 * n)
+        np.testing.assert_array_equal(x[0], 0.0)
+        np.testing.assert_array_equal(x[1], d)
+        np.testing.assert_allclose(x[2], yk, rtol=0, atol=1e-13)
+        np.testing.assert_array_equal(x[3], 0.0)
+        np.testing.assert_array_equal(v, np.array([[1.0] * n, [1.0] * n, [0.0] * n, [0.0] * n]))
+    s = onp.initialize_geodesics_at_camera(a, 60, d, lo, hi, n, camera_type="equator")
+    assert s.shape == (n, 8)
+    np.testing.assert_array_equal(s[:, 4], 1.0)
+    np.testing.assert_array_equal(s[:, 6:], 0.0)
+    assert np.all(s[:, 5] > 0)
+    for q in range(n):
+        g = onp.metric(s[q, :4], a)
+        k = s[q, 4:]
+        assert abs(k @ g @ k) < 1e-12

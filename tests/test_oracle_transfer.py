@@ -115,3 +115,31 @@ def test_golden_transfer_fixture_regenerates():
     I, dI = onp.solve_specific_intensity(f["jj"], f["aa"], f["dt"], float(f["L"]), dIs=True)
     np.testing.assert_array_equal(I, f["I"])
     np.testing.assert_array_equal(dI, f["dI"])
+
+
+def test_dIs_rows_are_in_scan_order_closed_form():
+    """solve_specific_intensity(dIs=True) returns the per-step increments in
+    the order of the reference's scan, i = T-1 down to 1 (transfer.py:111-117):
+    row k is the increment of sample i = T-1-k.  Without absorption that
+    increment is -dt[i-1] L j[i], distinct for every (i, pixel) here, so a
+    reversed, shifted or transposed row order fails; the rows sum to I."""
+    T, npx, L = 7, 3, 2.0
+    i = np.arange(T)[:, None]
+    p = np.arange(npx)[None, :]
+    j = (i + 1.0) + 0.1 * p                     # j[i, p]
+    dt = -(i + 1.0) * 0.01 - 0.001 * p          # dt[i, p]
+    I, dI = onp.solve_specific_intensity(j, np.zeros((T, npx)), dt, L, dIs=True)
+    assert dI.shape == (T - 1, npx)
+    for k in range(T - 1):
+        ii = T - 1 - k
+        np.testing.assert_array_equal(dI[k], -dt[ii - 1] * L * j[ii])
+    np.testing.assert_allclose(dI.sum(0), I, rtol=1e-15)
+    # with absorption, row k uses the intensity accumulated over rows < k
+    al = 0.3 + 0.01 * i + 0 * p
+    I2, dI2 = onp.solve_specific_intensity(j, al, dt, L, dIs=True)
+    acc = np.zeros(npx)
+    for k in range(T - 1):
+        ii = T - 1 - k
+        np.testing.assert_allclose(dI2[k], -dt[ii - 1] * L * (j[ii] - al[ii] * acc), rtol=1e-14)
+        acc = acc + dI2[k]
+    np.testing.assert_allclose(acc, I2, rtol=1e-14)
