@@ -71,7 +71,7 @@ def time_launches(L, fn, steps, warmup):
     return np.array(ms), wall
 
 
-def serve_stream(models, precision, seconds, rate, seed=0, log=None):
+def serve_stream(models, precision, seconds, rate, seed=0, log=None, persistent=False):
     """Poisson stream of image requests against resident snapshots through
     the request server: resolution uniform in {64, 128, 256, 512}^2, spin
     U[-0.98, 0.98], inclination U[5, 85] deg, frequency in {86, 230, 345} GHz,
@@ -88,7 +88,7 @@ def serve_stream(models, precision, seconds, rate, seed=0, log=None):
     inc = rng.uniform(5.0, 85.0, n_req)
     nu = rng.choice([86e9, 230e9, 345e9], n_req)
     snap = rng.integers(0, len(models), n_req)
-    with Server(precision, use_graph=True) as srv:
+    with Server(precision, use_graph=True, persistent=persistent) as srv:
         sids = [srv.add_snapshot(m) for m in models]
         # one small request first so that the first timed request does not
         # pay for lazy initialisation
@@ -118,14 +118,16 @@ def serve_stream(models, precision, seconds, rate, seed=0, log=None):
     out = {"workload": f"Poisson arrivals at {rate:g} req/s for {seconds:g} s ({n_req} image requests), "
                        "resolution uniform in {64,128,256,512}^2, spin U[-0.98,0.98], inclination U[5,85] deg, "
                        f"frequency in {{86,230,345}} GHz, {len(models)} resident 256^3-cell torus snapshots",
-           "precision": precision, "requests": n_req, "rays": int(rays), "wall_s": round(wall, 3),
+           "precision": precision, "form": "persistent launch" if persistent else "relaunched every 256 steps",
+           "requests": n_req, "rays": int(rays), "wall_s": round(wall, 3),
            "rays_per_s": round(rays / wall, 1), "offered_rays_per_s": round(rays / t_arr[-1], 1),
            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                           "max": round(float(lat.max()), 3)},
            "gpu_busy_frac": round((st["gpu_ms"] - st0["gpu_ms"]) / 1e3 / wall, 4),
            "launches": int(st["iterations"] - st0["iterations"])}
     if log:
-        print(f"[bench] serving {precision}: {out['rays_per_s'] / 1e6:.2f} M rays/s, p50 {out['latency_ms']['p50']} ms, "
+        print(f"[bench] serving {precision} ({out['form']}): {out['rays_per_s'] / 1e6:.2f} M rays/s, "
+              f"p50 {out['latency_ms']['p50']} ms, "
               f"p99 {out['latency_ms']['p99']} ms, busy {out['gpu_busy_frac']}", file=log, flush=True)
     return out
 
@@ -339,6 +341,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="length of the Poisson serving stream")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean requests/s of the serving stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--serve-form", choices=["relaunch", "persistent", "both"], default="relaunch",
+                    help="server form for the Poisson stream (both: an A/B of the two)")
     ap.add_argument("--no-heavy", action="store_true", help="skip the heavy queued stream (8 snapshots)")
     ap.add_argument("--no-target", action="store_true", help="skip the a=0.9 precision-target image")
     ap.add_argument("--no-sizes", action="store_true", help="skip the 512^2 and 1024^2 images")
@@ -463,6 +467,22 @@ def main():
             tr = json.load(f).get(head)
         if tr and tr.get("kernel") == roofline["kernel"]:
             roofline["traffic"] = tr["bytes_per_launch"]
+    # counter-derived VALU FLOP rate: SQ_INSTS_VALU_FLOPS_FP32 + _FP64 per
+    # launch (wave-level, FMA counted twice) x 64 lanes over this run's kernel
+    # time, from a PMC run of this same workload
+    ffile = os.path.join(ROOT, "profiles", "render_flops.json")
+    if os.path.exists(ffile):
+        with open(ffile) as f:
+            fl = json.load(f).get(head)
+        if fl and fl.get("kernel") == roofline["kernel"]:
+            flops = 64.0 * (fl["SQ_INSTS_VALU_FLOPS_FP32"] + fl["SQ_INSTS_VALU_FLOPS_FP64"])
+            ach = flops / (kern_ms * 1e-3) / 1e12
+            roofline["counters"] = {
+                "flop_per_launch": flops, "achieved": round(ach, 3), "frac": round(ach / peak, 4),
+                "frac_of_unpacked_peak": round(ach / (PEAK_FP32 / 2 if head != "plain64" else PEAK_FP64), 4),
+                "what": "SQ_INSTS_VALU_FLOPS_FP32 + SQ_INSTS_VALU_FLOPS_FP64 (x 64 lanes) per launch from "
+                        "profiles/render_flops.json over this run's HIP-event kernel time; the unpacked peak is "
+                        "the FP32 rate without packed FP32 (78.6 TFLOP/s, as FP64)"}
 
     out = {
         "metric": "rays/s (256^2 Kerr + RT), mixed vs plain FP64 at 1e-3 flux err; p50/p99 latency",
@@ -565,9 +585,12 @@ def main():
     if not args.no_serve:
         ds1 = uniform_torus_snapshot(a, seed=1)
         model1 = AthenakFluidModel.from_datasets(ds1, a, fluid_gamma=13. / 9)
-        sv = serve_stream([model, model1], head if head != "mixed-zone" else "mixed", args.serve_seconds,
-                          args.serve_rate, log=sys.stderr)
-        out["serving"] = sv
+        forms = {"relaunch": [False], "persistent": [True], "both": [False, True]}[args.serve_form]
+        for pers in forms:
+            sv = serve_stream([model, model1], head if head != "mixed-zone" else "mixed", args.serve_seconds,
+                              args.serve_rate, log=sys.stderr, persistent=pers)
+            out["serving_persistent" if pers and len(forms) > 1 else "serving"] = sv
+        sv = out["serving"]
         out["latency_ms"] = dict(sv["latency_ms"], what="submit-to-result latency of the serving stream")
         del model1, ds1
 

@@ -9,7 +9,7 @@
 //   2 / 3  geodesic step: RK4 + radius + step rule, FP64 / mixed (the
 //          geodesic wave's arithmetic per round, geodesics.py:249-261, 317-336)
 //   4 / 5  trilinear interpolation of the 8 primitives from 8 corner records
-//          held in registers, FP64 / FP32 (athenak.py:737-752)
+//          read from LDS, FP64 / FP32 (athenak.py:737-752)
 //   6 / 7  emission of one sample: fluid scalars, R(beta), j and alpha,
 //          FP64 / FP32 (athenak.py:759-794, images.py:86-118,
 //          transfer.py:30-86)
@@ -94,19 +94,30 @@ __global__ void __launch_bounds__(64) k_step_cycles(int variant, int steps, doub
     Stencil st;
     st.mb = 0;
     st.base = 0;
-    Corners cr;
+    // two sets of 8 corner records per lane in LDS, read every repetition
+    // (as the render kernel reads them from memory), so that nothing derived
+    // from the corners can be hoisted out of the loop
+    __shared__ float4 corners[2][16][64];
+    for (int set = 0; set < 2; ++set)
 #pragma unroll
-    for (int c8 = 0; c8 < 8; ++c8) {
-      const float w = 1.0f + 0.01f * c8 + 0.001f * lane;
-      cr.q[c8][0] = make_float4(w, 0.02f * w, 0.01f, 0.2f);
-      cr.q[c8][1] = make_float4(0.01f, 0.03f, 0.05f * w, 0.02f);
-    }
+      for (int c8 = 0; c8 < 8; ++c8) {
+        const float w = 1.0f + 0.01f * c8 + 0.001f * lane + 0.003f * set;
+        corners[set][2 * c8][lane] = make_float4(w, 0.02f * w, 0.01f, 0.2f);
+        corners[set][2 * c8 + 1][lane] = make_float4(0.01f, 0.03f, 0.05f * w, 0.02f);
+      }
+    __syncthreads();
+    Corners cr;
     double I = 0.0, trans = 1.0;
     pin(s[1]);
     c0 = __builtin_readcyclecounter();
     w0 = __builtin_amdgcn_s_memrealtime();
     for (int k = 0; k < steps; ++k) {
       const double f = 0.25 + 1e-5 * k;
+#pragma unroll
+      for (int c8 = 0; c8 < 8; ++c8) {
+        cr.q[c8][0] = corners[k & 1][2 * c8][lane];
+        cr.q[c8][1] = corners[k & 1][2 * c8 + 1][lane];
+      }
       st.fr[0] = f;
       st.fr[1] = 0.5 - 0.5 * f;
       st.fr[2] = 0.75 * f;

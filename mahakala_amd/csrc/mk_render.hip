@@ -460,6 +460,8 @@ __device__ __forceinline__ int wait_round(int* produced, int* done, int t, int* 
 // 2 only for launches without a zone, where that test never fires: the zone
 // code in the loop layout that measured fastest for launches of several
 // workgroup waves (builds 0 and 1 were 5-9 % slower there, same results).
+// Build 2's test is per ray, not per tile, so it must never see a zone:
+// launch_tri refuses one (MK_ERR_ARG).
 template <bool MIXED, bool LDS, int DB_, int ZONE>
 __global__ void __launch_bounds__(192 * kPairTiles) k_render_tri(SnapView sv, RenderDev rp, double* __restrict__ img,
                                                                   int32_t* __restrict__ steps, int tab_doubles,
@@ -814,7 +816,9 @@ std::vector<int> ring_first_tile_order(int res, double lo, double step, double a
 // launches use row-major tiles.
 namespace {
 std::mutex g_order_mu;
-std::map<std::tuple<int, int, double, double, double>, int*> g_orders;
+// device table and the host copy it was filled from (kept, so that the
+// stream-ordered copy never reads freed memory)
+std::map<std::tuple<int, int, double, double, double>, std::pair<int*, std::vector<int>>> g_orders;
 size_t g_order_bytes = 0;
 constexpr size_t kOrderCacheBytes = (size_t)64 << 20;
 }  // namespace
@@ -825,19 +829,20 @@ static const int* render_tile_order(const RenderDev& rp, const mk_render_params*
   const auto key = std::make_tuple(dev, rp.res, p->fov, p->bhspin, p->inclination);
   std::lock_guard<std::mutex> lk(g_order_mu);
   auto it = g_orders.find(key);
-  if (it != g_orders.end()) return it->second;
+  if (it != g_orders.end()) return it->second.first;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return nullptr;
   const size_t bytes = (size_t)rp.ntiles * sizeof(int);
   if (g_order_bytes + bytes > kOrderCacheBytes) return nullptr;
-  const std::vector<int> ord = ring_first_tile_order(rp.res, rp.lo, rp.step, p->bhspin, p->inclination);
+  std::vector<int> ord = ring_first_tile_order(rp.res, rp.lo, rp.step, p->bhspin, p->inclination);
   int* d = nullptr;
   if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
-  if (hipMemcpy(d, ord.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+  // ordered on the caller's stream before the launch that reads it
+  if (hipMemcpyAsync(d, ord.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess) {
     (void)hipFree(d);
     return nullptr;
   }
-  g_orders.emplace(key, d);
+  g_orders.emplace(key, std::make_pair(d, std::move(ord)));
   g_order_bytes += bytes;
   return d;
 }
@@ -903,11 +908,19 @@ static int launch_tri(const SnapView& sv, const RenderDev& rp, double* d_image, 
   nwg = (nwg + 7) / 8 * 8;
   const size_t lds = tab + tri_lds_extra<MIXED, DB_>();
   // strided tiles for the workgroups resident at once (see k_render_tri)
+  // resident workgroups of the build that runs: the zone build (1) for a
+  // zone; otherwise the zone-free build (0) when the grid is resident at
+  // once with it, else build 2
+  const bool zoned = MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0);
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_tri<MIXED, LDS, DB_, MIXED ? 1 : 0>,
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, zoned ? k_render_tri<MIXED, LDS, DB_, MIXED ? 1 : 0> : k_render_tri<MIXED, LDS, DB_, 0>,
+        192 * kPairTiles, lds);
+  if (e == hipSuccess && MIXED && !zoned && nwg > (long)per_cu * cus)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_tri<MIXED, LDS, DB_, MIXED ? 2 : 0>,
                                                      192 * kPairTiles, lds);
   if (e != hipSuccess) {
     set_error("mk_render: occupancy query failed");
@@ -920,10 +933,14 @@ static int launch_tri(const SnapView& sv, const RenderDev& rp, double* d_image, 
   // every workgroup is resident at once (-2 to -3 % at 192^2-256^2) and
   // build 2 in launches of several workgroup waves (see k_render_tri)
   const dim3 grid((unsigned)nwg), blk(192 * kPairTiles);
-  if (MIXED && (rp.fp64_r > 0.0 || rp.ring_r > 0.0))
+  if (zoned)
     k_render_tri<MIXED, LDS, DB_, MIXED ? 1 : 0><<<grid, blk, lds, st>>>(sv, rp, d_image, d_steps, tabd, nres);
-  else if (MIXED && nwg > R)
+  else if (MIXED && nwg > R) {
+    // build 2 decides FP64 steps per ray: a zone here would break the
+    // tile-wide rule every other form follows
+    MK_REQUIRE(rp.fp64_r <= 0.0 && rp.ring_r <= 0.0, "mk_render: internal: zone sent to the zone-free build");
     k_render_tri<MIXED, LDS, DB_, MIXED ? 2 : 0><<<grid, blk, lds, st>>>(sv, rp, d_image, d_steps, tabd, nres);
+  }
   else
     k_render_tri<MIXED, LDS, DB_, 0><<<grid, blk, lds, st>>>(sv, rp, d_image, d_steps, tabd, nres);
   MK_LAUNCH_CHECK();

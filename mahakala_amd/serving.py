@@ -29,11 +29,14 @@ class Server:
     """Owner of an ``mk_server`` (destroyed by ``close`` or on collection)."""
 
     def __init__(self, precision="plain64", lanes=0, steps_per_launch=0, max_requests=0, hbm_budget=0,
-                 use_graph=True):
+                 use_graph=True, persistent=False):
+        """persistent: one resident launch whose waves take newly admitted
+        work without a launch boundary (MK_SERVER_PERSISTENT), instead of a
+        launch per steps_per_launch steps."""
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         cfg = _lib.mk_server_config(PRECISIONS[precision], int(lanes), int(steps_per_launch), int(max_requests),
-                                    int(hbm_budget), int(bool(use_graph)), 0)
+                                    int(hbm_budget), int(bool(use_graph)), 1 if persistent else 0)
         h = _lib.vp()
         _lib.check(_lib.lib().mk_server_create(C.byref(cfg), C.byref(h)))
         self._h = h.value
