@@ -112,6 +112,8 @@ def serve_stream(models, precision, seconds, rate, seed=0, log=None, persistent=
         srv.wait(-1, timeout=600)
         wall = time.perf_counter() - t0
         done = dict(srv.poll(cap=n_req + 16))
+        if persistent:
+            time.sleep(0.1)  # the idle launch ends and posts its counters
         st = srv.stats()
     lat = np.array([done[i] for i in ids])
     rays = float(np.sum(res.astype(np.float64) ** 2))
@@ -124,6 +126,8 @@ def serve_stream(models, precision, seconds, rate, seed=0, log=None, persistent=
            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                           "max": round(float(lat.max()), 3)},
            "gpu_busy_frac": round((st["gpu_ms"] - st0["gpu_ms"]) / 1e3 / wall, 4),
+           "lane_utilisation": round((st["lane_steps_busy"] - st0["lane_steps_busy"])
+                                     / max(1, st["lane_steps_total"] - st0["lane_steps_total"]), 4),
            "launches": int(st["iterations"] - st0["iterations"])}
     if log:
         print(f"[bench] serving {precision} ({out['form']}): {out['rays_per_s'] / 1e6:.2f} M rays/s, "
@@ -139,7 +143,7 @@ def _heavy_snapshot(seed):
     return a, torus_snapshot(a, layout="uniform", seed=seed, turbulence_sigma=0.5)
 
 
-def heavy_stream(models, seed=1, n_images=128, n_shadow=128, res=512, n_angles=4096, log=None):
+def heavy_stream(models, seed=1, n_images=128, n_shadow=128, res=512, n_angles=4096, log=None, persistent=False):
     """256 queued requests against 8 resident snapshots: 512^2 images at 8
     frequencies log-spaced over 86-690 GHz (each image at its snapshot's spin
     and a random inclination) and 4096-angle shadow bisections at random spin
@@ -158,7 +162,7 @@ def heavy_stream(models, seed=1, n_images=128, n_shadow=128, res=512, n_angles=4
     angles = np.arange(n_angles) / n_angles * 2 * np.pi
     res_out, lat = {}, {}
     for precision in ("mixed", "plain64"):
-        with Server(precision, use_graph=True) as srv:
+        with Server(precision, use_graph=True, persistent=persistent) as srv:
             sids = [srv.add_snapshot(m) for m in models]
             srv.result(srv.submit_image(sids[0], 0.5, 30.0, resolution=16), timeout=120)
             srv.poll()
@@ -173,6 +177,8 @@ def heavy_stream(models, seed=1, n_images=128, n_shadow=128, res=512, n_angles=4
                     ids.append(srv.submit_shadow(float(spin[k]), float(inc[k]), angles))
             srv.wait(-1, timeout=900)
             wall = time.perf_counter() - t0
+            if persistent:
+                time.sleep(0.1)  # the idle launch ends and posts its counters
             st = srv.stats()
             done = dict(srv.poll(cap=kinds.size + 16))
             res_out[precision] = [srv.result(i, 1) for i in ids]
@@ -341,8 +347,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="length of the Poisson serving stream")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean requests/s of the serving stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
-    ap.add_argument("--serve-form", choices=["relaunch", "persistent", "both"], default="relaunch",
-                    help="server form for the Poisson stream (both: an A/B of the two)")
+    ap.add_argument("--serve-form", choices=["relaunch", "persistent", "both"], default="persistent",
+                    help="server form for the Poisson and heavy streams (both: an A/B of the two)")
     ap.add_argument("--no-heavy", action="store_true", help="skip the heavy queued stream (8 snapshots)")
     ap.add_argument("--no-target", action="store_true", help="skip the a=0.9 precision-target image")
     ap.add_argument("--no-sizes", action="store_true", help="skip the 512^2 and 1024^2 images")
@@ -602,7 +608,11 @@ def main():
         models8 = [AthenakFluidModel.from_datasets(d, a8, fluid_gamma=13. / 9) for a8, d in dss]  # uploads
         del dss
         print(f"[bench] 8 heavy snapshots ready in {time.perf_counter() - t1:.1f}s", file=sys.stderr, flush=True)
-        out["serving_heavy"] = heavy_stream(models8, log=sys.stderr)
+        forms = {"relaunch": [False], "persistent": [True], "both": [False, True]}[args.serve_form]
+        for pers in forms:
+            hv = heavy_stream(models8, log=sys.stderr, persistent=pers)
+            hv["form"] = "persistent launch" if pers else "relaunched every 256 steps"
+            out["serving_heavy_persistent" if pers and len(forms) > 1 else "serving_heavy"] = hv
         del models8
 
     if not args.no_cpu:

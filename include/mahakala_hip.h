@@ -248,8 +248,19 @@ typedef struct mk_server_config {
   int32_t max_requests;     /* requests in flight; 0 = 1024 */
   size_t hbm_budget;        /* device bytes for in-flight outputs; 0 = 4 GiB */
   int32_t use_graph;        /* replay the per-iteration launch as a hipGraph */
-  int32_t flags;            /* reserved, 0 */
+  int32_t flags;            /* MK_SERVER_* bits, 0 by default */
 } mk_server_config;
+
+/* mk_server_config.flags: one persistent launch instead of a launch per
+ * steps_per_launch steps.  Its waves take tiles in the order the worker
+ * publishes them, run each tile to the end and stay resident while work keeps
+ * coming; the launch ends once the server has been idle for 20 ms (or, after
+ * 1 s, whenever nothing is in flight) and the next request relaunches it.
+ * Work, descriptors and outputs pass through pinned host memory, and
+ * hbm_budget (default 1 GiB here) bounds the pinned output staging.  The
+ * results are the same bits as the relaunched form's and mk_render's.
+ * Statistics of this form are updated when a launch ends. */
+#define MK_SERVER_PERSISTENT 1
 
 typedef struct mk_image_request {
   int32_t snapshot;         /* id from mk_server_add_snapshot */
@@ -272,7 +283,8 @@ typedef struct mk_server_stats {
   double gpu_ms;            /* summed launch time (HIP events) */
   size_t hbm_in_use, hbm_budget;
   int64_t lane_steps_busy;  /* lane-steps that advanced a ray or bisection */
-  int64_t lane_steps_total; /* lanes x steps_per_launch x launches */
+  int64_t lane_steps_total; /* lanes x steps_per_launch x launches; persistent
+                               form: 64 x the wave-steps of every tile run */
   int32_t lanes, steps_per_launch; /* effective configuration */
 } mk_server_stats;
 

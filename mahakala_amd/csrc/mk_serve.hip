@@ -38,8 +38,9 @@ namespace mk {
 
 enum : int { kKindImage = 1, kKindShadow = 2 };
 
-// Device descriptor of one admitted request.
-struct ReqDev {
+// Device descriptor of one admitted request.  Whole 256-byte lines: a
+// descriptor never shares a cache line with another request's.
+struct alignas(256) ReqDev {
   int kind, n_items, res, N, iters, tiled;
   double a, ci, si, dist, lo, step, rEH, div, inv_div, tol;
   ImageParams ip;
@@ -49,6 +50,8 @@ struct ReqDev {
   const double* sang;
   const int* order;    // image (tiled): the tile taken k-th, longest expected first
 };
+
+static_assert(sizeof(ReqDev) % 256 == 0, "descriptor slots are whole cache lines");
 
 // A segment of the work ring: the tiles of one request (64 work items each:
 // an 8x8 pixel block, a run of 64 pixels, or 64 bisection angles).
@@ -184,7 +187,10 @@ __device__ __forceinline__ void consume(const ReqDev& q, const Pending& p, Lane&
 // only settles the pending sample (the wave is out of steps).  Returns true
 // when the ray is finished.  Keeping one call site for consume() keeps the
 // arithmetic of every sample the same wherever a launch boundary falls.
-template <bool MIXED>
+// IN_PLACE: the sample is consumed right after its fetch instead of after
+// the next step (the same arithmetic on the same values, so the same bits),
+// so no corner records are held across a step: fewer registers.
+template <bool MIXED, bool IN_PLACE = false>
 __device__ __forceinline__ bool image_pass(const ReqDev& q, bool go, Lane& L, Pending& p, bool& pend) {
   bool fin = false, adv = false;
   double nw[8];
@@ -222,6 +228,10 @@ __device__ __forceinline__ bool image_pass(const ReqDev& q, bool go, Lane& L, Pe
         for (int c = 0; c < 8; ++c) p.s[c] = nw[c];
         p.dt = L.dt;
         pend = true;
+        if constexpr (IN_PLACE) {
+          consume<MIXED>(q, p, L);
+          pend = false;
+        }
       }
 #pragma unroll
       for (int c = 0; c < 8; ++c) L.s[c] = nw[c];
@@ -377,6 +387,244 @@ __global__ void __launch_bounds__(256, 2) k_serve(const ReqDev* __restrict__ req
   }
 }
 
+// ------------------------------------------------------------ persistent form
+// One launch serves requests until the host asks it to stop: work admitted
+// while it runs starts without a launch boundary.  Tiles (64 work items) are
+// numbered by ticket in the order the host publishes them.  A wave takes the
+// next ticket with one fetch-and-add, published or not, waits until the host
+// has published it, runs the tile to the end and takes the next ticket.
+// (Claiming only published tickets needs a compare-and-swap loop; when work
+// appears, a thousand waiting waves retrying it cost about N^2/2 atomic round
+// trips, and the launch served a twentieth of the relaunched form's rate.)
+// While the launch runs the host issues no copy to or from device memory: it
+// publishes work in pinned, coherent host memory (segments, descriptors, tile
+// orders and angle tables) and the waves read it there; finished pixels and
+// radii go straight to pinned host memory, and the wave that completes a
+// request stores the request's sequence number in a pinned completion flag.
+// Every wait is bounded: a waiting wave sleeps between polls and exits when
+// the host sets `stop`, or reports an error after `idle_limit` sleeps.  The
+// host sets `stop` only when every published tile is done, and publishes
+// nothing more until the launch has ended, so the waves leave holding
+// unpublished tickets and the next launch resumes the count at the published
+// total.  Descriptor slots and input tables are fresh memory for every
+// request of a launch (the host relaunches before it reuses any), so no cache
+// of a running launch can hold an older request's copy of them.
+
+struct PSeg {          // one published request: tickets [start, start + count)
+  long long start;
+  int rid, count, seq, desc;
+};
+
+struct PCtl {          // pinned host memory; each word on its own 128-byte line
+  int stop, err, pad0[30];
+  long long pub_tiles;
+  long long pad1[15];
+  int pub_segs, pad2[31];
+};
+
+struct PArgs {
+  PCtl* ctl;                    // host
+  const PSeg* segs;             // host, seg_cap entries
+  int* fin;                     // host, per request slot
+  unsigned long long* taken;    // device: tickets handed out
+  unsigned long long* seen;     // device: largest published count a wave has read, | kHalt once stopping
+  int* done;                    // device: finished items per request slot
+  unsigned long long* busy;     // device: lane-steps that advanced a ray
+  unsigned long long* total;    // device: 64 x the wave-steps of every tile run
+  int seg_cap, idle_limit;
+};
+
+constexpr unsigned long long kHalt = 1ull << 62;  // in `seen`: the host has set stop
+constexpr long long kFrontTickets = 64;           // waiters this close to the published count poll host memory
+
+__device__ __forceinline__ long long bcast64(long long v) {
+  const int lo = __builtin_amdgcn_readfirstlane(__shfl((int)(v & 0xffffffffLL), 0));
+  const int hi = __builtin_amdgcn_readfirstlane(__shfl((int)(v >> 32), 0));
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Synchronisation without acquires.  On this GPU an acquire at agent or
+// system scope invalidates the L2 of the wave's XCD, and the busy waves of that
+// XCD would fetch their snapshot blocks from HBM again; the relaunched form's
+// loop has no acquire either.  Everything a wave reads that the host or
+// another wave wrote during the launch is read where no cache holds it:
+// segments, published counts, tile orders and angle tables are in pinned
+// coherent host memory, which the L2 does not cache, and are read with
+// system-scope loads; descriptors are read through the scalar cache, which the
+// launch started empty, from slots written before their segment was published
+// and never rewritten during the launch; the device counters are accessed with
+// agent-scope atomics, which are coherent across XCDs.  Program order orders a
+// wave's reads: each read is issued after the value it depends on has returned
+// (the segment after the published count, the descriptor after the segment).
+
+// Lane 0 waits until ticket t is published: 1 when it is, 0 when the host
+// has set stop, -1 after `idle_limit` sleeps.  The device word `seen` is read
+// every round; waiters within kFrontTickets of it also read host memory every
+// round (so new work is seen within one round), the others every 64 rounds.
+// A waiter that reads stop in host memory marks `seen` for all the others.
+__device__ __forceinline__ int wait_published(const PArgs& P, long long t) {
+  int state = 0;
+  if (__lane_id() == 0) {
+    for (int round = 0, slept = 0;; ++round) {
+      const unsigned long long v = __hip_atomic_load(P.seen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      long long pub = (long long)(v & ~kHalt);
+      if (t < pub) { state = 1; break; }
+      if (v & kHalt) break;
+      const long long d = t - pub;
+      if (d < kFrontTickets || ((round + (int)t) & 63) == 0) {
+        pub = __hip_atomic_load(&P.ctl->pub_tiles, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_fetch_max(P.seen, (unsigned long long)pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t < pub) { state = 1; break; }
+        if (__hip_atomic_load(&P.ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
+          __hip_atomic_fetch_or(P.seen, kHalt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+      // about 13 us a round near the published count, 54 us far from it
+      const int naps = d < 512 ? 4 : 16;
+      for (int k = 0; k < naps; ++k) __builtin_amdgcn_s_sleep(127);
+      slept += naps;
+      if (slept >= P.idle_limit) { state = -1; break; }
+    }
+  }
+  return __builtin_amdgcn_readfirstlane(__shfl(state, 0));
+}
+
+// Lane 0 finds the published segment holding ticket t (a binary search over
+// the last seg_cap segments, from the wave's cursor); false if none does.
+__device__ __forceinline__ bool find_segment(const PArgs& P, long long t, int& cursor, PSeg& out) {
+  int ok = 0;
+  PSeg e{};
+  if (__lane_id() == 0) {
+    const int nseg = __hip_atomic_load(&P.ctl->pub_segs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    int lo = cursor > nseg - P.seg_cap ? cursor : nseg - P.seg_cap, hi = nseg - 1;
+    if (lo < 0) lo = 0;
+    if (lo < hi) {  // most tickets lie in the wave's current segment or the next one
+      const PSeg* m = P.segs + (lo % P.seg_cap);
+      const int sq = __hip_atomic_load(&m->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      const long long st = __hip_atomic_load(&m->start, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      const int ct = __hip_atomic_load(&m->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      if (sq == lo && st <= t && t < st + ct) hi = lo;
+    }
+    // entries older than the ones in flight may be overwritten by newer
+    // ones (seq differs); they lie before t's segment
+    for (int guard = 0; guard < 40 && lo < hi; ++guard) {
+      const int mid = lo + (hi - lo + 1) / 2;
+      const PSeg* m = P.segs + (mid % P.seg_cap);
+      const int sq = __hip_atomic_load(&m->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      const long long st = __hip_atomic_load(&m->start, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      if (sq != mid || st <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    if (lo >= 0 && lo < nseg) {
+      const PSeg* m = P.segs + (lo % P.seg_cap);
+      e.seq = __hip_atomic_load(&m->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      e.start = __hip_atomic_load(&m->start, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      e.count = __hip_atomic_load(&m->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      e.rid = __hip_atomic_load(&m->rid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      e.desc = __hip_atomic_load(&m->desc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      ok = (e.seq == lo && e.start <= t && t < e.start + e.count) ? 1 : 0;
+      if (ok) cursor = lo;
+    }
+  }
+  ok = __builtin_amdgcn_readfirstlane(__shfl(ok, 0));
+  out.start = bcast64(e.start);
+  out.rid = __builtin_amdgcn_readfirstlane(__shfl(e.rid, 0));
+  out.count = __builtin_amdgcn_readfirstlane(__shfl(e.count, 0));
+  out.seq = __builtin_amdgcn_readfirstlane(__shfl(e.seq, 0));
+  out.desc = __builtin_amdgcn_readfirstlane(__shfl(e.desc, 0));
+  return ok != 0;
+}
+
+// `req` is the descriptor slots in pinned host memory, read as the relaunched
+// form reads its table: scalar loads through a read-only pointer.
+template <bool MIXED>
+__global__ void __launch_bounds__(256, 2) k_serve_persistent(const ReqDev* __restrict__ req, PArgs P) {
+  const int lane = threadIdx.x & 63;
+  int cursor = 0;
+  while (true) {
+    unsigned long long tk = 0;
+    if (lane == 0) tk = __hip_atomic_fetch_add(P.taken, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const long long t = bcast64((long long)tk);
+    const int state = wait_published(P, t);
+    if (state <= 0) {
+      if (state < 0 && lane == 0) __hip_atomic_store(&P.ctl->err, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      break;
+    }
+    PSeg sg;
+    if (!find_segment(P, t, cursor, sg)) {
+      if (lane == 0) __hip_atomic_store(&P.ctl->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      break;
+    }
+    const ReqDev& q = req[sg.desc];
+    const int tile = (int)(t - sg.start), rid = sg.rid;
+    const int item = tile * kTile + lane;
+    unsigned steps = 0;   // this tile's lane-steps, added to the busy count at its end
+    unsigned wsteps = 0;  // and its wave-steps
+    Lane L{};
+    L.cmb = -1;
+    bool live = item < q.n_items;
+    if (live && item_begin(q, item, L)) {
+      q.out[item] = L.f10;
+      live = false;
+    }
+    if (q.kind == kKindImage) {
+      // the tile order lives in host memory: read it once per tile
+      const int px = live ? pixel_of_item(q, item) : 0;
+      Pending p;
+      bool pend = false;
+      while (true) {
+        const bool go = __any(live);
+        wsteps += go;
+        if (live) {
+          steps += go;
+          if (image_pass<MIXED, true>(q, go, L, p, pend)) {
+            q.out[px] = L.f10;
+            live = false;
+          }
+        }
+        if (!go) break;
+      }
+    } else {
+      while (__any(live)) {
+        ++wsteps;
+        if (live) {
+          ++steps;
+          if (shadow_step(q, item, L)) {
+            q.out[item] = L.f10;
+            live = false;
+          }
+        }
+      }
+    }
+    unsigned long long w = steps;
+    for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
+    // The tile's outputs are complete in host memory (a system-scope release:
+    // write-back and wait, no invalidation) before its items are counted, so
+    // when the request's last tile sees the full count every output of the
+    // request is already there, and that wave's flag store follows it.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    if (lane == 0) {
+      atomicAdd(P.busy, w);
+      atomicAdd(P.total, 64ull * wsteps);
+      const int n = min(kTile, q.n_items - tile * kTile);
+      const int old = __hip_atomic_fetch_add(&P.done[rid], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (old + n >= q.n_items) {
+        __hip_atomic_store(&P.done[rid], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        __hip_atomic_store(&P.fin[rid], sg.seq + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+    }
+  }
+}
+
+__global__ void k_set_pair(unsigned long long* p, unsigned long long a, unsigned long long b) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    p[0] = a;
+    p[1] = b;
+  }
+}
+
 __global__ void k_fill_int(int* p, long n, int v) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -431,7 +679,25 @@ struct mk_server {
     Clock::time_point t0;
     std::vector<double> cs;  // shadow: cos, sin of the angles (copy source)
     std::vector<int> ord;    // image: tile order (copy source)
+    int seq = -1;            // persistent form: the request's segment number
   };
+  // persistent form (MK_SERVER_PERSISTENT): pinned, coherent host memory
+  // shared with the running launch, and its device counters
+  bool persistent = false;
+  PCtl* p_ctl = nullptr;
+  PSeg* p_segs = nullptr;
+  ReqDev* p_desc = nullptr;
+  int* p_fin = nullptr;
+  char* p_in = nullptr;              // tile orders and angle tables of a launch
+  size_t p_in_cap = 0, p_in_off = 0;
+  char* p_out = nullptr;             // output staging (the budgeted arena)
+  unsigned long long* d_p = nullptr; // tickets taken, published count seen, busy and total lane-steps
+  int* d_pdone = nullptr;
+  long long pub_tiles = 0;
+  long long resume_ticket = 0;       // the next launch's first ticket: the published count at the last stop
+  int seg_seq = 0, desc_seq = 0, desc_cap = 0;
+  bool launched = false, draining = false, stopping = false;
+  std::chrono::steady_clock::time_point launch_t0;
   std::map<std::pair<int, double>, std::vector<int>> orders;  // tile orders by (resolution, fov)
   std::deque<Pending> waiting;
   std::map<int, Flight> flight;
@@ -518,6 +784,67 @@ std::vector<int> tile_order(mk_server* s, int res, double fov, double lo, double
   return ord;
 }
 
+// Descriptor fields of a request, all but its memory (out, order, cang,
+// sang); cs receives a bisection's cos and sin table (cos then sin).
+void fill_desc(mk_server* s, const mk_server::Pending& p, ReqDev& q, std::vector<double>* cs) {
+  std::memset(&q, 0, sizeof q);
+  if (p.kind == kKindImage) {
+    mk_render_params rpar{};
+    const mk_image_request& r = p.img;
+    rpar.bhspin = r.bhspin;
+    rpar.inclination = r.inclination;
+    rpar.distance = r.distance;
+    rpar.fov = r.fov;
+    rpar.mass_scale = r.mass_scale;
+    rpar.M_bh = r.M_bh;
+    rpar.r_high = r.r_high;
+    rpar.fluid_gamma = r.fluid_gamma;
+    rpar.observing_frequency = r.observing_frequency;
+    rpar.div = 40.0;
+    rpar.tol = 1e-4;
+    rpar.fallback_pitch = M_PI / 3.0;
+    rpar.sigma_cut = 100.0;
+    rpar.resolution = r.resolution;
+    rpar.max_nsteps = r.max_nsteps;
+    rpar.precision = s->cfg.precision;
+    RenderDev rd;
+    fill_render_dev(&rpar, &rd);
+    q.kind = kKindImage;
+    q.n_items = r.resolution * r.resolution;
+    q.res = r.resolution;
+    q.tiled = (r.resolution % 8 == 0) ? 1 : 0;
+    q.N = r.max_nsteps;
+    q.a = rd.a; q.ci = rd.ci; q.si = rd.si; q.dist = rd.dist; q.lo = rd.lo; q.step = rd.step;
+    q.rEH = rd.rEH; q.div = rd.div; q.inv_div = rd.inv_div; q.tol = rd.tol;
+    q.ip = rd.ip;
+    q.sv = s->snaps[r.snapshot]->view();
+    return;
+  }
+  const mk_shadow_request& r = p.sh;
+  const int n = r.n_angles;
+  int iters = 0;
+  double inner = 0.5, outer = 10.0;
+  while (outer - inner > r.error_allowed && iters < r.max_it) {
+    inner = (outer - inner) / 2 + inner;
+    ++iters;
+  }
+  const double inc = r.inclination * M_PI / 180;
+  q.kind = kKindShadow;
+  q.n_items = n;
+  q.N = r.max_steps;
+  q.iters = iters;
+  q.a = r.bhspin;
+  q.ci = std::cos(inc);
+  q.si = std::sin(inc);
+  q.dist = r.distance;
+  q.rEH = 1.0 + std::sqrt(1.0 - r.bhspin * r.bhspin);
+  cs->resize(2 * (size_t)n);
+  for (int k = 0; k < n; ++k) {
+    (*cs)[k] = std::cos(p.angles[k]);
+    (*cs)[n + k] = std::sin(p.angles[k]);
+  }
+}
+
 // Moves waiting requests into flight while they fit (caller holds mu).
 bool admit(mk_server* s) {
   while (!s->waiting.empty() && !s->free_rids.empty() && s->tail - s->head < s->seg_cap) {
@@ -527,70 +854,17 @@ bool admit(mk_server* s) {
     const int rid = s->free_rids.back();
     s->free_rids.pop_back();
     ReqDev& q = s->h_req[rid];
-    std::memset(&q, 0, sizeof q);
-    mk_server::Flight f{p.id, 0, off, p.bytes, 0, p.h_out, p.t0, {}};
-    if (p.kind == kKindImage) {
-      mk_render_params rpar{};
-      const mk_image_request& r = p.img;
-      rpar.bhspin = r.bhspin;
-      rpar.inclination = r.inclination;
-      rpar.distance = r.distance;
-      rpar.fov = r.fov;
-      rpar.mass_scale = r.mass_scale;
-      rpar.M_bh = r.M_bh;
-      rpar.r_high = r.r_high;
-      rpar.fluid_gamma = r.fluid_gamma;
-      rpar.observing_frequency = r.observing_frequency;
-      rpar.div = 40.0;
-      rpar.tol = 1e-4;
-      rpar.fallback_pitch = M_PI / 3.0;
-      rpar.sigma_cut = 100.0;
-      rpar.resolution = r.resolution;
-      rpar.max_nsteps = r.max_nsteps;
-      rpar.precision = s->cfg.precision;
-      RenderDev rd;
-      fill_render_dev(&rpar, &rd);
-      q.kind = kKindImage;
-      q.n_items = r.resolution * r.resolution;
-      q.res = r.resolution;
-      q.tiled = (r.resolution % 8 == 0) ? 1 : 0;
-      q.N = r.max_nsteps;
-      q.a = rd.a; q.ci = rd.ci; q.si = rd.si; q.dist = rd.dist; q.lo = rd.lo; q.step = rd.step;
-      q.rEH = rd.rEH; q.div = rd.div; q.inv_div = rd.inv_div; q.tol = rd.tol;
-      q.ip = rd.ip;
-      q.sv = s->snaps[r.snapshot]->view();
-      q.out = (double*)(s->d_arena + off);
+    mk_server::Flight f{p.id, 0, off, p.bytes, 0, p.h_out, p.t0, {}, {}};
+    fill_desc(s, p, q, &f.cs);
+    q.out = (double*)(s->d_arena + off);
+    if (q.kind == kKindImage) {
       if (q.tiled) {
-        f.ord = tile_order(s, r.resolution, r.fov, rd.lo, rd.step);
-        q.order = (const int*)(s->d_arena + off + image_out_bytes(r.resolution));
+        f.ord = tile_order(s, p.img.resolution, p.img.fov, q.lo, q.step);
+        q.order = (const int*)(s->d_arena + off + image_out_bytes(p.img.resolution));
       }
     } else {
-      const mk_shadow_request& r = p.sh;
-      const int n = r.n_angles;
-      int iters = 0;
-      double inner = 0.5, outer = 10.0;
-      while (outer - inner > r.error_allowed && iters < r.max_it) {
-        inner = (outer - inner) / 2 + inner;
-        ++iters;
-      }
-      const double inc = r.inclination * M_PI / 180;
-      q.kind = kKindShadow;
-      q.n_items = n;
-      q.N = r.max_steps;
-      q.iters = iters;
-      q.a = r.bhspin;
-      q.ci = std::cos(inc);
-      q.si = std::sin(inc);
-      q.dist = r.distance;
-      q.rEH = 1.0 + std::sqrt(1.0 - r.bhspin * r.bhspin);
-      q.out = (double*)(s->d_arena + off);
-      f.cs.resize(2 * (size_t)n);
-      for (int k = 0; k < n; ++k) {
-        f.cs[k] = std::cos(p.angles[k]);
-        f.cs[n + k] = std::sin(p.angles[k]);
-      }
-      q.cang = (double*)(s->d_arena + off) + n;
-      q.sang = q.cang + n;
+      q.cang = (double*)(s->d_arena + off) + q.n_items;
+      q.sang = q.cang + q.n_items;
     }
     f.n_items = q.n_items;
     f.out_bytes = (size_t)q.n_items * sizeof(double);
@@ -731,6 +1005,238 @@ void worker_main(mk_server* s) {
   s->cv_done.notify_all();
 }
 
+// ---------------------------------------------------------- persistent form
+// The host ends a launch once nothing has been queued or in flight for
+// kPersistIdleMs, or, when a launch older than kPersistMaxAgeMs finds nothing
+// in flight, at once: a waiting wave then holds a ticket for at most that long
+// plus the longest request.  A wave reports an error after kPersistWaitSleeps
+// sleeps of about 3.4 us (about 28 s).
+constexpr int kPersistIdleMs = 20;
+constexpr int kPersistMaxAgeMs = 1000;
+constexpr int kPersistWaitSleeps = 1 << 23;
+constexpr size_t kPersistInputBytes = (size_t)256 << 20;  // tile orders and angle tables per launch
+constexpr int kPersistDescSlots = 16384;          // descriptor slots per launch
+
+template <typename T>
+void store_shared(T* p, T v) {
+  __atomic_store_n(p, v, __ATOMIC_RELEASE);
+}
+template <typename T>
+T load_shared(const T* p) {
+  return __atomic_load_n(p, __ATOMIC_ACQUIRE);
+}
+
+// Publishes waiting requests to the running (or next) launch while they fit
+// (caller holds mu).  A request whose tables or descriptor would not fit the
+// launch's fresh memory starts a drain: nothing more is admitted until every
+// request in flight has finished and the launch has ended.
+void admit_persistent(mk_server* s) {
+  if (s->launched && s->stopping) return;  // the waves may be leaving: nothing is published until the launch ends
+  bool any = false;
+  while (!s->draining && !s->waiting.empty() && !s->free_rids.empty()) {
+    mk_server::Pending& p = s->waiting.front();
+    size_t in_bytes = 0;
+    if (p.kind == kKindImage) {
+      const int res = p.img.resolution;
+      in_bytes = (res % 8 == 0) ? align_up((size_t)(res / 8) * (res / 8) * sizeof(int)) : 0;
+    } else {
+      in_bytes = align_up((size_t)2 * p.sh.n_angles * sizeof(double));
+    }
+    if (s->p_in_off + in_bytes > s->p_in_cap || s->desc_seq >= s->desc_cap) {
+      s->draining = true;
+      break;
+    }
+    const size_t out_bytes = align_up((size_t)(p.kind == kKindImage ? p.img.resolution * p.img.resolution
+                                                                     : p.sh.n_angles) * sizeof(double));
+    size_t off;
+    if (!arena_alloc(s, out_bytes, &off)) break;  // back-pressure
+    const int rid = s->free_rids.back();
+    s->free_rids.pop_back();
+    mk_server::Flight f{p.id, 0, off, out_bytes, 0, p.h_out, p.t0, {}, {}};
+    ReqDev q;
+    fill_desc(s, p, q, &f.cs);
+    q.out = (double*)(s->p_out + off);
+    char* in = s->p_in + s->p_in_off;
+    if (q.kind == kKindImage) {
+      if (q.tiled) {
+        const std::vector<int>& ord = tile_order(s, p.img.resolution, p.img.fov, q.lo, q.step);
+        std::memcpy(in, ord.data(), ord.size() * sizeof(int));
+        q.order = (const int*)in;
+      }
+    } else {
+      std::memcpy(in, f.cs.data(), f.cs.size() * sizeof(double));
+      q.cang = (const double*)in;
+      q.sang = q.cang + q.n_items;
+      f.cs.clear();
+    }
+    s->p_in_off += in_bytes;
+    const int slot = s->desc_seq++;
+    std::memcpy(&s->p_desc[slot], &q, sizeof q);
+    f.n_items = q.n_items;
+    f.out_bytes = (size_t)q.n_items * sizeof(double);
+    f.seq = s->seg_seq;
+    store_shared(&s->p_fin[rid], 0);
+    PSeg& e = s->p_segs[s->seg_seq % s->seg_cap];
+    e.start = s->pub_tiles;
+    e.rid = rid;
+    e.count = (q.n_items + kTile - 1) / kTile;
+    e.desc = slot;
+    store_shared(&e.seq, s->seg_seq);
+    s->pub_tiles += e.count;
+    ++s->seg_seq;
+    s->flight[rid] = std::move(f);
+    s->waiting.pop_front();
+    ++s->st.admitted;
+    any = true;
+  }
+  if (any) {  // the segment count first: a wave that sees a ticket sees its segment
+    store_shared(&s->p_ctl->pub_segs, s->seg_seq);
+    store_shared(&s->p_ctl->pub_tiles, s->pub_tiles);
+  }
+}
+
+// Finished requests: their completion flag holds seq + 1 (caller holds mu).
+void collect_persistent(mk_server* s) {
+  std::vector<int> fin;
+  for (auto& kv : s->flight)
+    if (load_shared(&s->p_fin[kv.first]) == kv.second.seq + 1) fin.push_back(kv.first);
+  for (int rid : fin) {
+    const mk_server::Flight& f = s->flight[rid];
+    std::memcpy(f.h_out, s->p_out + f.off, f.out_bytes);
+    const double ms = std::chrono::duration<double, std::milli>(Clock::now() - f.t0).count();
+    const int64_t id = f.id;
+    const int n_items = f.n_items;
+    arena_free(s, f.off, f.bytes);
+    s->flight.erase(rid);
+    s->free_rids.push_back(rid);
+    if (s->unpolled.size() >= kMaxUnpolled)
+      s->unpolled.erase(s->unpolled.begin(), s->unpolled.begin() + kMaxUnpolled / 2);
+    s->unpolled.emplace_back(id, ms);
+    ++s->st.completed;
+    s->st.items_done += n_items;
+  }
+  if (!fin.empty()) s->cv_done.notify_all();
+}
+
+bool launch_persistent(mk_server* s) {
+  PArgs P;
+  P.ctl = s->p_ctl;
+  P.segs = s->p_segs;
+  P.fin = s->p_fin;
+  P.taken = s->d_p;
+  P.seen = s->d_p + 1;
+  P.busy = s->d_p + 2;
+  P.total = s->d_p + 3;
+  P.done = s->d_pdone;
+  P.seg_cap = s->seg_cap;
+  P.idle_limit = kPersistWaitSleeps;
+  store_shared(&s->p_ctl->stop, 0);
+  s->stopping = false;
+  const unsigned blocks = (unsigned)((s->L + 255) / 256);
+  // tickets below resume_ticket are done; the previous launch's waves left
+  // holding tickets from it on, unpublished then
+  k_set_pair<<<1, 64, 0, s->stream>>>(s->d_p, (unsigned long long)s->resume_ticket,
+                                      (unsigned long long)s->resume_ticket);
+  SRV_HIP(hipGetLastError());
+  SRV_HIP(hipEventRecord(s->ev0, s->stream));
+  if (s->cfg.precision == 0) k_serve_persistent<false><<<blocks, 256, 0, s->stream>>>(s->p_desc, P);
+  else k_serve_persistent<true><<<blocks, 256, 0, s->stream>>>(s->p_desc, P);
+  SRV_HIP(hipGetLastError());
+  SRV_HIP(hipEventRecord(s->ev1, s->stream));
+  s->launched = true;
+  s->launch_t0 = Clock::now();
+  ++s->st.iterations;
+  return true;
+}
+
+// Asks the running launch to end; called only when every published tile is
+// done (caller holds mu).  Nothing is published until the launch has ended.
+void stop_persistent(mk_server* s) {
+  if (s->stopping) return;
+  s->stopping = true;
+  s->resume_ticket = s->pub_tiles;
+  store_shared(&s->p_ctl->stop, 1);
+}
+
+// The launch has ended: its time and lane-steps go to the statistics.
+bool end_persistent(mk_server* s) {
+  float ms = 0;
+  SRV_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  unsigned long long c[2] = {0, 0};  // busy and total lane-steps, over every launch so far
+  SRV_HIP(hipMemcpy(c, s->d_p + 2, sizeof c, hipMemcpyDeviceToHost));
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->st.gpu_ms += ms;
+  s->st.lane_steps_busy = (int64_t)c[0];
+  s->st.lane_steps_total = (int64_t)c[1];
+  s->launched = false;
+  return true;
+}
+
+void worker_persistent(mk_server* s) {
+  if (hipSetDevice(s->dev) != hipSuccess) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->status = MK_ERR_HIP;
+    s->err = "worker: hipSetDevice failed";
+    s->cv_done.notify_all();
+    return;
+  }
+  bool ok = true;
+  Clock::time_point last_busy = Clock::now();
+  while (ok && !s->stop.load()) {
+    if (s->launched) {
+      const hipError_t q = hipEventQuery(s->ev1);
+      if (q == hipSuccess) ok = end_persistent(s);
+      else if (q != hipErrorNotReady) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->status = MK_ERR_HIP;
+        s->err = std::string("persistent launch: ") + hipGetErrorString(q);
+        break;
+      }
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (const int ec = load_shared(&s->p_ctl->err)) {
+      s->status = MK_ERR_STATE;
+      s->err = ec == 1 ? "persistent launch: a ticket without a published segment"
+                       : "persistent launch: a wave waited longer than its limit for work";
+      break;
+    }
+    collect_persistent(s);
+    if (s->draining && s->flight.empty()) {
+      if (s->launched) {
+        stop_persistent(s);  // the launch ends within about a tenth of a millisecond
+      } else {  // the next launch starts with fresh descriptor slots and tables
+        s->p_in_off = 0;
+        s->desc_seq = 0;
+        s->draining = false;
+      }
+    }
+    admit_persistent(s);
+    const Clock::time_point now = Clock::now();
+    if (!s->flight.empty() || !s->waiting.empty()) {
+      last_busy = now;
+    }
+    if (s->launched && s->flight.empty() &&
+        ((s->waiting.empty() && now - last_busy > std::chrono::milliseconds(kPersistIdleMs)) ||
+         now - s->launch_t0 > std::chrono::milliseconds(kPersistMaxAgeMs))) {
+      stop_persistent(s);  // nothing in flight: the next request relaunches
+    }
+    if (!s->launched && !s->flight.empty()) {
+      lk.unlock();
+      ok = launch_persistent(s);
+      last_busy = Clock::now();
+      continue;
+    }
+    if (s->flight.empty() && s->waiting.empty() && !s->launched)
+      s->cv_work.wait_for(lk, std::chrono::milliseconds(20));
+    else
+      s->cv_work.wait_for(lk, std::chrono::microseconds(50));
+  }
+  store_shared(&s->p_ctl->stop, 1);  // the waves finish every published tile, then leave
+  (void)hipStreamSynchronize(s->stream);
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->cv_done.notify_all();
+}
+
 int server_error(mk_server* s) {
   if (s->status == MK_OK) return MK_OK;
   set_error("mk_server: %s", s->err.c_str());
@@ -755,9 +1261,56 @@ int mk_server_create(const mk_server_config* cfg, mk_server** out) {
   s->L = (s->L + 255) / 256 * 256;  // whole blocks: every wave is full
   s->M = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : 256;
   s->Rmax = cfg->max_requests > 0 ? cfg->max_requests : 1024;
-  s->arena_bytes = cfg->hbm_budget > 0 ? cfg->hbm_budget : ((size_t)4 << 30);
+  s->persistent = (cfg->flags & MK_SERVER_PERSISTENT) != 0;
+  // the persistent form stages outputs in pinned host memory: 1 GiB by default
+  s->arena_bytes = cfg->hbm_budget > 0 ? cfg->hbm_budget : ((size_t)(s->persistent ? 1 : 4) << 30);
   const long L = s->L;
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (s->persistent) {
+    // seg_cap > Rmax: a segment's entry is overwritten only long after its
+    // request finished (at most Rmax requests are in flight)
+    s->seg_cap = std::max(4096, 4 * s->Rmax);
+    s->desc_cap = kPersistDescSlots;
+    s->p_in_cap = kPersistInputBytes;
+    const unsigned fl = hipHostMallocCoherent | hipHostMallocMapped;
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->p_ctl, sizeof(PCtl), fl);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->p_segs, s->seg_cap * sizeof(PSeg), fl);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->p_desc, s->desc_cap * sizeof(ReqDev), fl);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->p_fin, s->Rmax * sizeof(int), fl);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->p_in, s->p_in_cap, fl);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->p_out, s->arena_bytes, fl);
+    if (e == hipSuccess) e = hipMalloc(&s->d_p, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&s->d_pdone, s->Rmax * sizeof(int));
+    // the launch dereferences these host pointers: they must be its addresses too
+    for (void* h : {(void*)s->p_ctl, (void*)s->p_segs, (void*)s->p_desc, (void*)s->p_fin, (void*)s->p_in,
+                    (void*)s->p_out}) {
+      void* d = nullptr;
+      if (e == hipSuccess) e = hipHostGetDevicePointer(&d, h, 0);
+      if (e == hipSuccess && d != h) e = hipErrorInvalidValue;
+    }
+    if (e == hipSuccess) {
+      std::memset(s->p_ctl, 0, sizeof(PCtl));
+      std::memset(s->p_segs, 0xff, s->seg_cap * sizeof(PSeg));  // seq -1: no segment yet
+      std::memset(s->p_fin, 0, s->Rmax * sizeof(int));
+      k_fill_int<<<1, 64, 0, s->stream>>>((int*)s->d_p, 8, 0);
+      k_fill_int<<<grid_for(s->Rmax, 256), 256, 0, s->stream>>>(s->d_pdone, s->Rmax, 0);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventCreate(&s->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+      set_error("mk_server_create: %s", hipGetErrorString(e));
+      mk_server_destroy(s);
+      return e == hipErrorOutOfMemory ? MK_ERR_NOMEM : MK_ERR_HIP;
+    }
+    s->free_blocks[0] = s->arena_bytes;
+    for (int r = s->Rmax - 1; r >= 0; --r) s->free_rids.push_back(r);
+    s->st.hbm_budget = s->arena_bytes;
+    s->worker = std::thread(worker_persistent, s);
+    *out = s;
+    return MK_OK;
+  }
   if (e == hipSuccess) e = hipMalloc(&s->d_req, s->Rmax * sizeof(ReqDev));
   if (e == hipSuccess) e = hipMalloc(&s->d_done, s->Rmax * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&s->d_segs, s->seg_cap * sizeof(Seg));
@@ -928,6 +1481,11 @@ int mk_server_destroy(mk_server* s) {
   (void)hipFree(s->d_f);
   (void)hipFree(s->d_i);
   (void)hipFree(s->d_arena);
+  (void)hipFree(s->d_p);
+  (void)hipFree(s->d_pdone);
+  for (void* h : {(void*)s->p_ctl, (void*)s->p_segs, (void*)s->p_desc, (void*)s->p_fin, (void*)s->p_in,
+                  (void*)s->p_out})
+    if (h) (void)hipHostFree(h);
   if (s->h_req) (void)hipHostFree(s->h_req);
   if (s->h_segs) (void)hipHostFree(s->h_segs);
   if (s->h_done) (void)hipHostFree(s->h_done);

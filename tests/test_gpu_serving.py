@@ -7,8 +7,12 @@ requests; images.py:30-144 and geodesics.py:405-435 of the reference).
 * served images equal mk_render's and served bisections equal
   mk_shadow_bisect's (same arithmetic per ray);
 * admission respects the HBM budget (back-pressure, never-fitting requests
-  rejected) and wait() times out.
+  rejected) and wait() times out;
+* the persistent form gives the same bits, and its launch ends when the
+  server is idle and resumes its ticket count on the next request.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -45,18 +49,21 @@ def _render(model, res, spin, inc, nu, precision):
     return images.render(model.device_snapshot, p)
 
 
+@pytest.mark.parametrize("persistent", [False, True])
 @pytest.mark.parametrize("precision", ["plain64", "mixed"])
-def test_served_images_equal_render_and_do_not_depend_on_the_mix(models, precision):
-    print(f"[serving] precision={precision}", flush=True)
+def test_served_images_equal_render_and_do_not_depend_on_the_mix(models, precision, persistent):
+    print(f"[serving] precision={precision} persistent={persistent}", flush=True)
     alone = {}
     # few lanes and short launches: rays of many requests share waves and
-    # each slot is reused many times
-    with Server(precision, lanes=2048, steps_per_launch=64) as srv:
+    # each slot is reused many times (the persistent form: one launch whose
+    # waves take tiles of every request as they are published)
+    kw = dict(lanes=2048, steps_per_launch=64, persistent=persistent)
+    with Server(precision, **kw) as srv:
         sids = [srv.add_snapshot(m) for m in models]
         for k, (res, spin, inc, nu, s) in enumerate(MIX):
             rid = srv.submit_image(sids[s], **_img_kw(res, spin, inc, nu))
             alone[k] = srv.result(rid, timeout=120)
-    with Server(precision, lanes=2048, steps_per_launch=64) as srv:
+    with Server(precision, **kw) as srv:
         sids = [srv.add_snapshot(m) for m in models]
         ang = np.linspace(0, 2 * np.pi, 24, endpoint=False)
         sh = srv.submit_shadow(0.3, 40.0, ang, max_steps=1500)
@@ -65,6 +72,7 @@ def test_served_images_equal_render_and_do_not_depend_on_the_mix(models, precisi
         srv.result(sh, timeout=120)
         st = srv.stats()
     assert st["completed"] == len(MIX) + 1 and st["in_flight"] == 0 and st["hbm_in_use"] == 0
+    assert st["iterations"] >= 1
     for k, (res, spin, inc, nu, s) in enumerate(MIX):
         d = np.abs(alone[k] - mixed[k])
         print(f"[serving] request {k}: alone vs mixed max abs diff {d.max():.3e} at {np.unravel_index(d.argmax(), d.shape)}")
@@ -82,11 +90,12 @@ def test_served_images_equal_render_and_do_not_depend_on_the_mix(models, precisi
             assert abs(alone[k].sum() - ref64.sum()) <= 1e-3 * ref64.sum()
 
 
-def test_served_shadow_equals_bisection_kernel():
+@pytest.mark.parametrize("persistent", [False, True])
+def test_served_shadow_equals_bisection_kernel(persistent):
     ang = np.linspace(0, 2 * np.pi, 40, endpoint=False)
     ref_a = find_shadow_bisection_angles(0.9, 60.0, ang, max_steps=2000)
     ref_b = find_shadow_bisection_angles(-0.4, 20.0, ang[::3], max_steps=1500, error_allowed=1e-2)
-    with Server("mixed", lanes=1024, steps_per_launch=128) as srv:
+    with Server("mixed", lanes=1024, steps_per_launch=128, persistent=persistent) as srv:
         a = srv.submit_shadow(0.9, 60.0, ang, max_steps=2000)
         b = srv.submit_shadow(-0.4, 20.0, ang[::3], max_steps=1500, error_allowed=1e-2)
         c = srv.submit_shadow(0.1, 45.0, ang[:5], error_allowed=20.0)  # no halving: 0.5
@@ -99,12 +108,31 @@ def test_served_shadow_equals_bisection_kernel():
     assert np.all((ref_a > 3) & (ref_a < 7))
 
 
-def test_budget_back_pressure_and_errors(models):
+def test_persistent_launch_ends_when_idle_and_resumes_its_tickets():
+    """The persistent launch ends after the server has been idle; its waves
+    leave holding unpublished tickets, and the next launch resumes at the
+    published count, so requests after each relaunch are served in full."""
+    ang = np.linspace(0, 2 * np.pi, 200, endpoint=False)  # 4 tiles, the last one partial
+    ref = find_shadow_bisection_angles(0.7, 50.0, ang, max_steps=1500)
+    with Server("mixed", lanes=4096, persistent=True) as srv:
+        for k in range(4):
+            print(f"[serving] persistent relaunch round {k}", flush=True)
+            rid = srv.submit_shadow(0.7, 50.0, ang, max_steps=1500)
+            np.testing.assert_array_equal(srv.result(rid, 60), ref)
+            time.sleep(0.1)  # longer than the idle limit: the launch ends
+        st = srv.stats()
+    assert st["completed"] == 4 and st["in_flight"] == 0
+    assert st["iterations"] >= 4  # one launch per round
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_budget_back_pressure_and_errors(models, persistent):
     res = 48
     # one request: its output (256-byte aligned already) and its tile order
-    # (36 tiles x 4 bytes, aligned to 256)
+    # (36 tiles x 4 bytes, aligned to 256; the persistent form keeps the
+    # order outside the budget)
     need = res * res * 8 + 256
-    with Server("mixed", hbm_budget=need, lanes=4096) as srv:
+    with Server("mixed", hbm_budget=need, lanes=4096, persistent=persistent) as srv:
         sid = srv.add_snapshot(models[0])
         ref = _render(models[0], res, 0.5, 30.0, 230e9, "mixed")
         rids = [srv.submit_image(sid, **_img_kw(res, 0.5, 30.0, 230e9)) for _ in range(3)]
@@ -123,7 +151,7 @@ def test_budget_back_pressure_and_errors(models):
         srv.wait(rids[0], timeout=0.0)  # finished and polled: returns at once
         with pytest.raises(_lib.MkError, match="unknown request"):
             srv.wait(10 ** 6, timeout=0.0)
-    with Server("plain64") as srv:
+    with Server("plain64", persistent=persistent) as srv:
         sid = srv.add_snapshot(models[0])
         rid = srv.submit_image(sid, **_img_kw(256, 0.5, 30.0, 230e9))
         with pytest.raises(_lib.MkError, match="timed out"):

@@ -48,7 +48,9 @@ def heavy():
 
 
 def _serve(heavy, precision, angles):
-    with Server(precision, lanes=8192, steps_per_launch=128) as srv:
+    # plain FP64 through the persistent form, mixed through the relaunched one
+    # (the two forms give the same bits)
+    with Server(precision, lanes=8192, steps_per_launch=128, persistent=(precision == "plain64")) as srv:
         sid = {s: srv.add_snapshot(m) for s, m in heavy.items()}
         rids = []
         for k, (s, inc, nu) in enumerate(VIEWS):
