@@ -352,6 +352,8 @@ def main():
     ap.add_argument("--no-heavy", action="store_true", help="skip the heavy queued stream (8 snapshots)")
     ap.add_argument("--no-target", action="store_true", help="skip the a=0.9 precision-target image")
     ap.add_argument("--no-sizes", action="store_true", help="skip the 512^2 and 1024^2 images")
+    ap.add_argument("--no-octree", action="store_true",
+                    help="skip the 1024^2 image on the 3-level turbulent octree (a=0.94, i=17 deg)")
     ap.add_argument("--no-shadow", action="store_true", help="skip the shadow bisection timings")
     ap.add_argument("--form", choices=["auto", "classic", "paired", "triple"], default="auto",
                     help="render kernel form (auto: the library's choice)")
@@ -539,6 +541,43 @@ def main():
             del img2
         out["larger_images"] = dict(sizes, what="same scene and precision as value, one launch per image")
         print(f"[bench] larger images: {sizes}", file=sys.stderr, flush=True)
+
+    if not args.no_octree:
+        # 1024^2 at a = 0.94, i = 17 deg, 230 GHz through the 512-leaf 3-level
+        # octree with log-normal turbulence: each precision timed, and the
+        # mixed presets' error against plain FP64
+        from mahakala_amd.grmhd.synthetic import octree_turbulent_snapshot
+        t1 = time.perf_counter()
+        mo = AthenakFluidModel.from_datasets(octree_turbulent_snapshot(0.94, seed=0), 0.94, fluid_gamma=13. / 9)
+        print(f"[bench] octree snapshot ready in {time.perf_counter() - t1:.1f}s", file=sys.stderr, flush=True)
+        ro = 1024
+        imgo = L.DeviceBuffer(8 * ro * ro)
+        oc, Io = {}, {}
+        for pr in ("plain64", "mixed", "mixed-zone"):
+            po = images.render_params(0.94, camera_inclination=17.0, resolution=ro, observing_frequency=230e9,
+                                      precision=pr)
+
+            def fno(po=po):
+                L.check(L.lib().mk_render(mo.device_snapshot.handle, po, imgo.ptr, None, None))
+            mso, _ = time_launches(L, fno, max(3, args.steps // 10), 2)
+            Io[pr] = imgo.to_array((ro, ro))
+            oc[pr] = {"ms_per_image": round(float(np.mean(mso)), 3),
+                      "rays_per_s": round(ro * ro / (float(np.mean(mso)) * 1e-3), 1)}
+        fo = float(Io["plain64"].sum())
+        bro = Io["plain64"] > 1e-3 * Io["plain64"].max()
+        for pr in ("mixed", "mixed-zone"):
+            d = np.abs(Io[pr] - Io["plain64"])
+            oc[pr].update(speedup_over_plain=round(oc["plain64"]["ms_per_image"] / oc[pr]["ms_per_image"], 3),
+                          flux_rel_err=abs(float(Io[pr].sum()) - fo) / fo,
+                          pixel_l1_rel=float(d.sum() / Io["plain64"].sum()),
+                          pixel_max_rel_bright=float((d[bro] / Io["plain64"][bro]).max()),
+                          ring_diameter=ring_diameter(Io[pr], 20.0))
+        oc["plain64"]["ring_diameter"] = ring_diameter(Io["plain64"], 20.0)
+        out["octree_1024"] = dict(oc, what="1024^2, a=0.94, i=17 deg, 230 GHz, fov 20 M, 512-leaf 3-level octree "
+                                           "over +-50 M with log-normal turbulence (sigma 0.5, seed 0)")
+        print(f"[bench] octree 1024^2: " + ", ".join(f"{k} {v['ms_per_image']} ms" for k, v in oc.items()),
+              file=sys.stderr, flush=True)
+        del mo, imgo
 
     if not args.no_shadow:
         # find_shadow_bisection(0.9, 60, 64) and a 4096-angle job: 14 halvings,

@@ -203,33 +203,43 @@ __device__ __forceinline__ void rk4_step(const T s[8], T dt, T a, T out[8]) {
 // Mixed step: stages in FP32, increment added to the FP64 state.  The stage
 // slopes stay unscaled (t = s + (h/2) F is one FMA, the slope sum is
 // F1 + 2 F2 + 2 F3 + F4 and h/6 scales it once), which saves the 32 products
-// k = h F of the scaled form.
+// k = h F of the scaled form.  Coordinate time (component 0) enters no
+// right-hand side and no image, so this step carries it over unchanged.
 __device__ __forceinline__ void rk4_step_mixed(const double s[8], double dt, float a, double out[8]) {
   float sf[8];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) sf[i] = (float)s[i];
+  for (int i = 1; i < 8; ++i) sf[i] = (float)s[i];
   const float h = (float)dt, hh = 0.5f * h, h6 = h * (1.0f / 6.0f);
   float F[8], acc[4], t[8], sum[8];
   ks_accel<float>(sf[1], sf[2], sf[3], sf[4], sf[5], sf[6], sf[7], a, acc);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) { F[i] = sf[4 + i]; F[4 + i] = acc[i]; }
+  for (int i = 1; i < 4; ++i) F[i] = sf[4 + i];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) { sum[i] = F[i]; t[i] = sf[i] + hh * F[i]; }
+  for (int i = 0; i < 4; ++i) F[4 + i] = acc[i];
+#pragma unroll
+  for (int i = 1; i < 8; ++i) { sum[i] = F[i]; t[i] = sf[i] + hh * F[i]; }
   ks_accel<float>(t[1], t[2], t[3], t[4], t[5], t[6], t[7], a, acc);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) { F[i] = t[4 + i]; F[4 + i] = acc[i]; }
+  for (int i = 1; i < 4; ++i) F[i] = t[4 + i];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) { sum[i] += 2.0f * F[i]; t[i] = sf[i] + hh * F[i]; }
+  for (int i = 0; i < 4; ++i) F[4 + i] = acc[i];
+#pragma unroll
+  for (int i = 1; i < 8; ++i) { sum[i] += 2.0f * F[i]; t[i] = sf[i] + hh * F[i]; }
   ks_accel<float>(t[1], t[2], t[3], t[4], t[5], t[6], t[7], a, acc);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) { F[i] = t[4 + i]; F[4 + i] = acc[i]; }
+  for (int i = 1; i < 4; ++i) F[i] = t[4 + i];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) { sum[i] += 2.0f * F[i]; t[i] = sf[i] + h * F[i]; }
+  for (int i = 0; i < 4; ++i) F[4 + i] = acc[i];
+#pragma unroll
+  for (int i = 1; i < 8; ++i) { sum[i] += 2.0f * F[i]; t[i] = sf[i] + h * F[i]; }
   ks_accel<float>(t[1], t[2], t[3], t[4], t[5], t[6], t[7], a, acc);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) { F[i] = t[4 + i]; F[4 + i] = acc[i]; }
+  for (int i = 1; i < 4; ++i) F[i] = t[4 + i];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) out[i] = s[i] + (double)(h6 * (sum[i] + F[i]));
+  for (int i = 0; i < 4; ++i) F[4 + i] = acc[i];
+  out[0] = s[0];
+#pragma unroll
+  for (int i = 1; i < 8; ++i) out[i] = s[i] + (double)(h6 * (sum[i] + F[i]));
 }
 
 // geodesics.py:284-291
