@@ -580,6 +580,28 @@ def main():
         del mo, imgo
 
     if not args.no_shadow:
+        # config 1's grid: 64x64 camera rays at a = 0.9, i = 60 deg integrated
+        # with the reference's step rule (N = 10000, div 40, tol = 0.01 r_EH)
+        # to their final accepted state, then the termination radius per pixel
+        from mahakala_amd import geodesics as G
+        a1 = 0.9
+        tol1 = 0.01 * (1 + np.sqrt(1 - a1 * a1))
+        s1 = np.asarray(G.initialize_geodesics_at_camera(a1, 60, 1000, -10, 10, 64))
+        n1 = s1.shape[0]
+        d1 = L.DeviceBuffer.from_array(s1)
+        dK1, dl1, dr1 = L.DeviceBuffer(4 * n1), L.DeviceBuffer(64 * n1), L.DeviceBuffer(8 * n1)
+
+        def fn1():
+            L.check(L.lib().mk_integrate_final(d1.ptr, n1, 10000, 40.0, tol1, a1, dK1.ptr, dl1.ptr, None, None))
+            L.check(L.lib().mk_radius_cal(dl1.ptr, n1, 8, a1, dr1.ptr, None))
+        ms1, _ = time_launches(L, fn1, 10, 2)
+        K1 = dK1.to_array((n1,), np.int32)
+        out["config1_grid"] = {"ms": round(float(np.mean(ms1)), 3), "rays_per_s": round(n1 / (float(np.mean(ms1)) * 1e-3), 1),
+                               "accepted_steps": int(np.minimum(K1, 10000).sum()), "max_steps_per_ray": int(K1.max()),
+                               "what": "64x64 camera rays, a=0.9, i=60 deg, camera at 1000 M, N=10000, div 40, "
+                                       "tol 0.01 r_EH: final accepted state and termination radius per pixel (FP64)"}
+        print(f"[bench] config 1 grid 64^2: {np.mean(ms1):.3f} ms", file=sys.stderr, flush=True)
+        del d1, dK1, dl1, dr1
         # find_shadow_bisection(0.9, 60, 64) and a 4096-angle job: 14 halvings,
         # each a full integration, reported as ray-integrations/s
         from mahakala_amd.geodesics import find_shadow_bisection_angles
@@ -667,6 +689,16 @@ def main():
                                "sample": f"the full {res}x{res} image once, C FP64 restatement with OpenMP "
                                          f"({dt_cpu:.1f} s wall)",
                                "flux_rel_err_vs_gpu_plain": abs(float(I_cpu.sum()) - flux_p) / flux_p}
+        if "config1_grid" in out:
+            # config 1's grid on the host cores: the C FP64 restatement of the
+            # same integration; its frozen rows must equal the GPU's
+            t1 = time.perf_counter()
+            Kc1, _, _ = c_oracle.integrate_final(s1, 10000, 40.0, tol1, a1, nthreads=nth)
+            dt1 = time.perf_counter() - t1
+            out["cpu_baseline_config1"] = {
+                "value": round(n1 / dt1, 1), "unit": "rays/s", "cores": nth, "kind": "port",
+                "sample": f"config 1's whole 64x64 grid once, C FP64 restatement with OpenMP ({dt1:.2f} s wall)",
+                "frozen_rows_equal_gpu": bool(np.array_equal(Kc1, K1))}
         # the reference's own dataflow: the NumPy FP64 restatement, a scan over
         # steps vectorised over rays (jacfwd by dual numbers, LU inverse), the
         # materialised history and the host meshblock-mask sampler, on one core

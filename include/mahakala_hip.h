@@ -216,6 +216,10 @@ typedef struct mk_render_params {
 /* Hand tiles out in row-major order instead of photon ring first (the same
  * image, bit for bit; only the schedule differs). */
 #define MK_RENDER_ROW_MAJOR 512
+/* Classic form: by default the launch holds only the workgroups resident at
+ * once and each wave takes the next tile from a counter until none is left;
+ * this bit launches one wave per tile instead (the same image, bit for bit). */
+#define MK_RENDER_STATIC_TILES 1024
 
 /* make_image  (images.py:30-144) for a device-resident snapshot: camera,
  * geodesic, sampling, emission and transfer fused per ray; d_image

@@ -14,6 +14,7 @@ struct RenderDev {
   double fp64_r, ring_r, ring_drdl;
   int spin_limit;     // polls before a bounded wait gives up (paired and triple forms)
   const int* order;   // tiles in the order they are handed out (nullptr: row-major)
+  int* next_tile;     // classic form: tiles taken from this counter by each wave (nullptr: one tile per wave)
 };
 
 int fill_render_dev(const mk_render_params* p, RenderDev* rp);

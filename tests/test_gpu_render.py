@@ -17,6 +17,7 @@ from tests import helpers
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 MSUN = 1.989e33
+MK_RENDER_STATIC_TILES = 1024  # include/mahakala_hip.h
 
 
 def _pixel_close(img, ref, rtol=1e-4):
@@ -176,7 +177,8 @@ def _form_of(model, params):
 @pytest.mark.parametrize("precision", ["plain64", "mixed", "mixed-zone"])
 @pytest.mark.parametrize("res", [24, 36, 256, 320])
 def test_kernel_forms_agree_bit_for_bit(torus, precision, res):
-    """The classic form (one wave per tile), the paired form (geodesic and
+    """The classic form (one wave per tile, tiles taken from a counter or one
+    launch slot each), the paired form (geodesic and
     transfer waves, samples handed over through LDS) and the triple form
     (geodesic, gather and emission waves, two LDS rings) evaluate the same
     expressions per ray in the same order: same image and same step counts,
@@ -188,9 +190,13 @@ def test_kernel_forms_agree_bit_for_bit(torus, precision, res):
         p = images.render_params(0.5, camera_inclination=30, resolution=res, precision=precision, form=form)
         assert _form_of(torus, p) == form
         out[form] = images.render(torus.device_snapshot, p, return_steps=True)
+    # the classic form with one wave per tile instead of tiles from a counter
+    p = images.render_params(0.5, camera_inclination=30, resolution=res, precision=precision, form="classic")
+    p.flags |= MK_RENDER_STATIC_TILES
+    out["classic-static"] = images.render(torus.device_snapshot, p, return_steps=True)
     a, sa = out["classic"]
     assert np.all(np.isfinite(a)) and a.max() > 0
-    for form in ("paired", "triple"):
+    for form in ("paired", "triple", "classic-static"):
         b, sb = out[form]
         d = np.abs(a - b)
         print(f"[render] {precision} {res}^2 {form} vs classic: max abs diff {d.max():.2e}, "
