@@ -191,6 +191,8 @@ __device__ __forceinline__ void stage_tables(const SnapView& sv, double* lds, co
 // stages the tables once, and each of its waves takes the next tile (in tile
 // order) from the counter until none is left, so a wave whose tile ends early
 // starts another instead of idling until its workgroup's longest tile ends.
+// next_tile[1] counts the waves that found none left; the last of them sets
+// both words back to 0, so the next launch on the stream starts from 0.
 template <bool MIXED, bool LDS, bool IN_PLACE>
 __global__ void __launch_bounds__(256, IN_PLACE ? 3 : 2) k_render(SnapView sv, RenderDev rp, double* __restrict__ img,
                                                                   int32_t* __restrict__ steps) {
@@ -204,7 +206,15 @@ __global__ void __launch_bounds__(256, IN_PLACE ? 3 : 2) k_render(SnapView sv, R
       int k = 0;
       if (lane == 0) k = atomicAdd(rp.next_tile, 1);
       k = __builtin_amdgcn_readfirstlane(__shfl(k, 0));
-      if (k >= rp.ntiles) break;
+      if (k >= rp.ntiles) {
+        // every other wave has taken its last ticket before it counts itself
+        // here, so the last one to count leaves no taker behind
+        if (lane == 0 && atomicAdd(rp.next_tile + 1, 1) == (int)(gridDim.x * (blockDim.x >> 6)) - 1) {
+          __hip_atomic_store(rp.next_tile, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(rp.next_tile + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        break;
+      }
       const long tile = rp.order ? rp.order[k] : k;
       const int i = (int)(tile / rp.tiles_x) * 8 + (lane >> 3), j = (int)(tile % rp.tiles_x) * 8 + (lane & 7);
       if (i < rp.res && j < rp.res) {
@@ -849,14 +859,11 @@ size_t g_order_bytes = 0;
 constexpr size_t kOrderCacheBytes = (size_t)64 << 20;
 }  // namespace
 
-// One tile counter per (device, stream) for the classic form's queue, zeroed
-// on the stream before each launch that uses it (allocated on first use).
+// One tile counter (two ints: next tile, waves done) per (device, stream) for
+// the classic form's queue: allocated and zeroed on the stream on first use,
+// and left at zero by every launch that uses it.
 static std::mutex g_counter_mu;
 static std::map<std::pair<int, hipStream_t>, int*> g_counters;
-
-__global__ void k_zero_counter(int* c) {
-  if (threadIdx.x == 0) *c = 0;
-}
 
 static int* tile_counter(hipStream_t st) {
   int dev = 0;
@@ -867,7 +874,11 @@ static int* tile_counter(hipStream_t st) {
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return nullptr;
   int* c = nullptr;
-  if (hipMalloc(&c, sizeof(int)) != hipSuccess) return nullptr;
+  if (hipMalloc(&c, 2 * sizeof(int)) != hipSuccess) return nullptr;
+  if (hipMemsetAsync(c, 0, 2 * sizeof(int), st) != hipSuccess) {
+    (void)hipFree(c);
+    return nullptr;
+  }
   g_counters.emplace(std::make_pair(dev, st), c);
   return c;
 }
@@ -1104,8 +1115,6 @@ extern "C" int mk_render(const mk_snapshot* snap, const mk_render_params* p, dou
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)block.x, kl) == hipSuccess && per_cu > 0) {
       const long resident = (long)per_cu * cus;
       if (resident < nwg) g = dim3((unsigned)resident);
-      k_zero_counter<<<1, 64, 0, st>>>(rp.next_tile);
-      MK_LAUNCH_CHECK();
     } else {
       rp.next_tile = nullptr;
     }

@@ -205,6 +205,36 @@ def test_kernel_forms_agree_bit_for_bit(torus, precision, res):
         np.testing.assert_array_equal(a, b)
 
 
+def test_classic_tile_counter_starts_from_zero_every_launch(torus):
+    """The classic form's waves take tiles from a per-stream counter that the
+    last wave of each launch resets: back-to-back launches on the null stream
+    and on a created stream (a counter of its own) all give the full image."""
+    import ctypes as C
+    from mahakala_amd import _lib
+    res = 512
+    p = images.render_params(0.5, camera_inclination=30, resolution=res, precision="mixed")
+    assert _form_of(torus, p) == "classic"
+    lib = _lib.lib()
+    st = C.c_void_p()
+    _lib.check(lib.mk_stream_create(C.byref(st)))
+    try:
+        ref = None
+        for k, stream in enumerate([None, None, st.value, st.value, None]):
+            buf = _lib.DeviceBuffer(8 * res * res)
+            _lib.check(lib.mk_memset(buf.ptr, 0, 8 * res * res, stream))
+            _lib.check(lib.mk_render(torus.device_snapshot.handle, p, buf.ptr, None, stream))
+            _lib.check(lib.mk_stream_sync(stream))
+            img = buf.to_array((res, res))
+            print(f"[render] classic launch {k} on {'null' if stream is None else 'created'} stream: "
+                  f"{np.count_nonzero(img)} nonzero pixels")
+            if ref is None:
+                ref = img
+                assert np.count_nonzero(img) > res * res // 4
+            np.testing.assert_array_equal(img, ref)
+    finally:
+        _lib.check(lib.mk_stream_destroy(st.value))
+
+
 @pytest.fixture(scope="module")
 def torus_many_blocks():
     """The torus as 4096 blocks of 8^3 cells: block tables of 393 KB, more than
