@@ -752,10 +752,24 @@ void arena_free(mk_server* s, size_t off, size_t bytes) {
   }
 }
 
+// A failed HIP call sets the server's error and returns false: SRV_HIP where
+// the caller holds mu (or no other thread runs yet), SRV_HIP_UNLOCKED where it
+// does not (the worker's waits on the GPU run without mu, so that submit and
+// stats calls are not blocked).
 #define SRV_HIP(call)                                                      \
   do {                                                                     \
     hipError_t e_ = (call);                                                \
     if (e_ != hipSuccess) {                                                \
+      s->status = MK_ERR_HIP;                                              \
+      s->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+      return false;                                                        \
+    }                                                                      \
+  } while (0)
+#define SRV_HIP_UNLOCKED(call)                                             \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess) {                                                \
+      std::lock_guard<std::mutex> lk_(s->mu);                              \
       s->status = MK_ERR_HIP;                                              \
       s->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
       return false;                                                        \
@@ -916,9 +930,9 @@ bool enqueue_iteration(mk_server* s) {
   const unsigned blocks = (unsigned)((s->L + 255) / 256);
   if (s->cfg.precision == 0) k_serve<false><<<blocks, 256, 0, s->stream>>>(s->d_req, A);
   else k_serve<true><<<blocks, 256, 0, s->stream>>>(s->d_req, A);
-  SRV_HIP(hipGetLastError());
-  SRV_HIP(hipMemcpyAsync(s->h_done, s->d_done, s->Rmax * sizeof(int), hipMemcpyDeviceToHost, s->stream));
-  SRV_HIP(hipMemcpyAsync(s->h_done + s->Rmax, s->d_head, 4 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  SRV_HIP_UNLOCKED(hipGetLastError());
+  SRV_HIP_UNLOCKED(hipMemcpyAsync(s->h_done, s->d_done, s->Rmax * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  SRV_HIP_UNLOCKED(hipMemcpyAsync(s->h_done + s->Rmax, s->d_head, 4 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
   return true;
 }
 
@@ -935,16 +949,17 @@ bool build_graph(mk_server* s) {
 }
 
 bool iterate(mk_server* s) {
-  SRV_HIP(hipEventRecord(s->ev0, s->stream));
+  SRV_HIP_UNLOCKED(hipEventRecord(s->ev0, s->stream));
   if (s->graph) {
-    SRV_HIP(hipGraphLaunch(s->graph, s->stream));
+    SRV_HIP_UNLOCKED(hipGraphLaunch(s->graph, s->stream));
   } else if (!enqueue_iteration(s)) {
     return false;
   }
-  SRV_HIP(hipEventRecord(s->ev1, s->stream));
-  SRV_HIP(hipEventSynchronize(s->ev1));
+  SRV_HIP_UNLOCKED(hipEventRecord(s->ev1, s->stream));
+  SRV_HIP_UNLOCKED(hipEventSynchronize(s->ev1));
   float ms = 0;
-  SRV_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  SRV_HIP_UNLOCKED(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  std::lock_guard<std::mutex> lk(s->mu);
   s->st.gpu_ms += ms;
   ++s->st.iterations;
   s->st.lane_steps_total += (int64_t)s->L * s->M;
@@ -1137,14 +1152,15 @@ bool launch_persistent(mk_server* s) {
   // holding tickets from it on, unpublished then
   k_set_pair<<<1, 64, 0, s->stream>>>(s->d_p, (unsigned long long)s->resume_ticket,
                                       (unsigned long long)s->resume_ticket);
-  SRV_HIP(hipGetLastError());
-  SRV_HIP(hipEventRecord(s->ev0, s->stream));
+  SRV_HIP_UNLOCKED(hipGetLastError());
+  SRV_HIP_UNLOCKED(hipEventRecord(s->ev0, s->stream));
   if (s->cfg.precision == 0) k_serve_persistent<false><<<blocks, 256, 0, s->stream>>>(s->p_desc, P);
   else k_serve_persistent<true><<<blocks, 256, 0, s->stream>>>(s->p_desc, P);
-  SRV_HIP(hipGetLastError());
-  SRV_HIP(hipEventRecord(s->ev1, s->stream));
+  SRV_HIP_UNLOCKED(hipGetLastError());
+  SRV_HIP_UNLOCKED(hipEventRecord(s->ev1, s->stream));
   s->launched = true;
   s->launch_t0 = Clock::now();
+  std::lock_guard<std::mutex> lk(s->mu);
   ++s->st.iterations;
   return true;
 }
@@ -1161,9 +1177,9 @@ void stop_persistent(mk_server* s) {
 // The launch has ended: its time and lane-steps go to the statistics.
 bool end_persistent(mk_server* s) {
   float ms = 0;
-  SRV_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  SRV_HIP_UNLOCKED(hipEventElapsedTime(&ms, s->ev0, s->ev1));
   unsigned long long c[2] = {0, 0};  // busy and total lane-steps, over every launch so far
-  SRV_HIP(hipMemcpy(c, s->d_p + 2, sizeof c, hipMemcpyDeviceToHost));
+  SRV_HIP_UNLOCKED(hipMemcpy(c, s->d_p + 2, sizeof c, hipMemcpyDeviceToHost));
   std::lock_guard<std::mutex> lk(s->mu);
   s->st.gpu_ms += ms;
   s->st.lane_steps_busy = (int64_t)c[0];
