@@ -537,9 +537,13 @@ __device__ __forceinline__ bool find_segment(const PArgs& P, long long t, int& c
 }
 
 // `req` is the descriptor slots in pinned host memory, read as the relaunched
-// form reads its table: scalar loads through a read-only pointer.
-template <bool MIXED>
-__global__ void __launch_bounds__(256, 2) k_serve_persistent(const ReqDev* __restrict__ req, PArgs P) {
+// form reads its table: scalar loads through a read-only pointer.  SHADOW:
+// the launch serves bisections too; without them the mixed image path fits
+// three waves per SIMD (168 VGPRs against 236: the bisection's FP64 walk
+// needs the rest), and the host publishes no bisection to such a launch.
+template <bool MIXED, bool SHADOW>
+__global__ void __launch_bounds__(256, (MIXED && !SHADOW) ? 3 : 2)
+    k_serve_persistent(const ReqDev* __restrict__ req, PArgs P) {
   const int lane = threadIdx.x & 63;
   int cursor = 0;
   while (true) {
@@ -585,7 +589,7 @@ __global__ void __launch_bounds__(256, 2) k_serve_persistent(const ReqDev* __res
         }
         if (!go) break;
       }
-    } else {
+    } else if constexpr (SHADOW) {
       while (__any(live)) {
         ++wsteps;
         if (live) {
@@ -596,6 +600,8 @@ __global__ void __launch_bounds__(256, 2) k_serve_persistent(const ReqDev* __res
           }
         }
       }
+    } else if (lane == 0) {  // the host publishes no bisection to this launch
+      __hip_atomic_store(&P.ctl->err, 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     unsigned long long w = steps;
     for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
@@ -680,6 +686,7 @@ struct mk_server {
     std::vector<double> cs;  // shadow: cos, sin of the angles (copy source)
     std::vector<int> ord;    // image: tile order (copy source)
     int seq = -1;            // persistent form: the request's segment number
+    bool shadow = false;     // persistent form: a bisection
   };
   // persistent form (MK_SERVER_PERSISTENT): pinned, coherent host memory
   // shared with the running launch, and its device counters
@@ -697,6 +704,10 @@ struct mk_server {
   long long resume_ticket = 0;       // the next launch's first ticket: the published count at the last stop
   int seg_seq = 0, desc_seq = 0, desc_cap = 0;
   bool launched = false, draining = false, stopping = false;
+  bool default_lanes = true;         // lanes not set by the caller: two waves per SIMD
+  bool launch_shadow = false;        // the running launch serves bisections too
+  bool launch_deep = false;          // the next launch starts with a deep backlog
+  bool want_shadow = false;          // a bisection waits for a launch that serves it
   std::chrono::steady_clock::time_point launch_t0;
   std::map<std::pair<int, double>, std::vector<int>> orders;  // tile orders by (resolution, fov)
   std::deque<Pending> waiting;
@@ -1027,6 +1038,13 @@ void worker_main(mk_server* s) {
 // plus the longest request.  A wave reports an error after kPersistWaitSleeps
 // sleeps of about 3.4 us (about 28 s).
 constexpr int kPersistIdleMs = 20;
+// A mixed launch without bisections runs three waves per SIMD when the work
+// queued or in flight is at least this many 64-item tiles per wave of the
+// default two-waves-per-SIMD launch.
+constexpr int kDeepTilesPerWave = 4;
+// A launch about to start without such a backlog first waits up to this long
+// for more work (a batch being submitted); only idle-to-busy starts pay it.
+constexpr int kPersistGatherUs = 200;
 constexpr int kPersistMaxAgeMs = 1000;
 constexpr int kPersistWaitSleeps = 1 << 23;
 constexpr size_t kPersistInputBytes = (size_t)256 << 20;  // tile orders and angle tables per launch
@@ -1050,6 +1068,12 @@ void admit_persistent(mk_server* s) {
   bool any = false;
   while (!s->draining && !s->waiting.empty() && !s->free_rids.empty()) {
     mk_server::Pending& p = s->waiting.front();
+    if (p.kind != kKindImage && s->launched && !s->launch_shadow) {
+      // a launch without the bisection path: hold the queue here (FIFO)
+      // until that launch has drained and ended
+      s->want_shadow = true;
+      break;
+    }
     size_t in_bytes = 0;
     if (p.kind == kKindImage) {
       const int res = p.img.resolution;
@@ -1068,6 +1092,7 @@ void admit_persistent(mk_server* s) {
     const int rid = s->free_rids.back();
     s->free_rids.pop_back();
     mk_server::Flight f{p.id, 0, off, out_bytes, 0, p.h_out, p.t0, {}, {}};
+    f.shadow = p.kind != kKindImage;
     ReqDev q;
     fill_desc(s, p, q, &f.cs);
     q.out = (double*)(s->p_out + off);
@@ -1147,15 +1172,27 @@ bool launch_persistent(mk_server* s) {
   P.idle_limit = kPersistWaitSleeps;
   store_shared(&s->p_ctl->stop, 0);
   s->stopping = false;
-  const unsigned blocks = (unsigned)((s->L + 255) / 256);
+  s->want_shadow = false;
+  bool shadow = false;
+  for (const auto& kv : s->flight) shadow = shadow || kv.second.shadow;
+  s->launch_shadow = shadow;
+  // mixed without bisections, default lanes and a deep backlog: three waves
+  // per SIMD, for throughput; otherwise two, each wave faster, for latency
+  unsigned blocks = (unsigned)((s->L + 255) / 256);
+  if (s->cfg.precision == 1 && !shadow && s->default_lanes && s->launch_deep) blocks = blocks * 3 / 2;
   // tickets below resume_ticket are done; the previous launch's waves left
   // holding tickets from it on, unpublished then
   k_set_pair<<<1, 64, 0, s->stream>>>(s->d_p, (unsigned long long)s->resume_ticket,
                                       (unsigned long long)s->resume_ticket);
   SRV_HIP_UNLOCKED(hipGetLastError());
   SRV_HIP_UNLOCKED(hipEventRecord(s->ev0, s->stream));
-  if (s->cfg.precision == 0) k_serve_persistent<false><<<blocks, 256, 0, s->stream>>>(s->p_desc, P);
-  else k_serve_persistent<true><<<blocks, 256, 0, s->stream>>>(s->p_desc, P);
+  if (s->cfg.precision == 0) {
+    if (shadow) k_serve_persistent<false, true><<<blocks, 256, 0, s->stream>>>(s->p_desc, P);
+    else k_serve_persistent<false, false><<<blocks, 256, 0, s->stream>>>(s->p_desc, P);
+  } else {
+    if (shadow) k_serve_persistent<true, true><<<blocks, 256, 0, s->stream>>>(s->p_desc, P);
+    else k_serve_persistent<true, false><<<blocks, 256, 0, s->stream>>>(s->p_desc, P);
+  }
   SRV_HIP_UNLOCKED(hipGetLastError());
   SRV_HIP_UNLOCKED(hipEventRecord(s->ev1, s->stream));
   s->launched = true;
@@ -1197,7 +1234,8 @@ void worker_persistent(mk_server* s) {
     return;
   }
   bool ok = true;
-  Clock::time_point last_busy = Clock::now();
+  Clock::time_point last_busy = Clock::now(), ready_since{};
+  bool ready = false;  // work admitted, no launch yet (since ready_since)
   while (ok && !s->stop.load()) {
     if (s->launched) {
       const hipError_t q = hipEventQuery(s->ev1);
@@ -1212,8 +1250,9 @@ void worker_persistent(mk_server* s) {
     std::unique_lock<std::mutex> lk(s->mu);
     if (const int ec = load_shared(&s->p_ctl->err)) {
       s->status = MK_ERR_STATE;
-      s->err = ec == 1 ? "persistent launch: a ticket without a published segment"
-                       : "persistent launch: a wave waited longer than its limit for work";
+      s->err = ec == 1   ? "persistent launch: a ticket without a published segment"
+               : ec == 4 ? "persistent launch: a bisection reached a launch without the bisection path"
+                         : "persistent launch: a wave waited longer than its limit for work";
       break;
     }
     collect_persistent(s);
@@ -1232,15 +1271,27 @@ void worker_persistent(mk_server* s) {
       last_busy = now;
     }
     if (s->launched && s->flight.empty() &&
-        ((s->waiting.empty() && now - last_busy > std::chrono::milliseconds(kPersistIdleMs)) ||
+        (s->want_shadow || (s->waiting.empty() && now - last_busy > std::chrono::milliseconds(kPersistIdleMs)) ||
          now - s->launch_t0 > std::chrono::milliseconds(kPersistMaxAgeMs))) {
       stop_persistent(s);  // nothing in flight: the next request relaunches
     }
     if (!s->launched && !s->flight.empty()) {
-      lk.unlock();
-      ok = launch_persistent(s);
-      last_busy = Clock::now();
-      continue;
+      long long items = 0;  // queued or in flight, counted here under mu
+      for (const auto& kv : s->flight) items += kv.second.n_items;
+      for (const auto& p : s->waiting)
+        items += p.kind == kKindImage ? (long long)p.img.resolution * p.img.resolution : p.sh.n_angles;
+      s->launch_deep = items >= (long long)kDeepTilesPerWave * s->L;
+      if (!ready) {
+        ready = true;
+        ready_since = now;
+      }
+      if (s->launch_deep || now - ready_since >= std::chrono::microseconds(kPersistGatherUs)) {
+        ready = false;
+        lk.unlock();
+        ok = launch_persistent(s);
+        last_busy = Clock::now();
+        continue;
+      }
     }
     if (s->flight.empty() && s->waiting.empty() && !s->launched)
       s->cv_work.wait_for(lk, std::chrono::milliseconds(20));
@@ -1273,6 +1324,7 @@ int mk_server_create(const mk_server_config* cfg, mk_server** out) {
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   s->dev = dev;
+  s->default_lanes = cfg->lanes <= 0;
   s->L = cfg->lanes > 0 ? cfg->lanes : std::max(cus, 1) * 4 * 2 * 64;
   s->L = (s->L + 255) / 256 * 256;  // whole blocks: every wave is full
   s->M = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : 256;

@@ -125,6 +125,32 @@ def test_persistent_launch_ends_when_idle_and_resumes_its_tickets():
     assert st["iterations"] >= 4  # one launch per round
 
 
+def test_persistent_bisection_arriving_during_an_image_only_launch(models):
+    """Default lanes, mixed: a launch that serves only images runs three waves
+    per SIMD without the bisection path.  A bisection submitted while it runs
+    waits until that launch has drained; the next launch serves both kinds.
+    Every result is the same as when served alone."""
+    ang = np.linspace(0, 2 * np.pi, 100, endpoint=False)
+    ref_sh = find_shadow_bisection_angles(0.6, 40.0, ang, max_steps=1500)
+    kws = [_img_kw(256, 0.5, 30.0, 230e9), _img_kw(192, -0.3, 60.0, 86e9), _img_kw(128, 0.8, 20.0, 345e9)]
+    refs = [_render(models[0], kw["resolution"], kw["bhspin"], kw["inclination"], kw["observing_frequency"], "mixed")
+            for kw in kws]
+    with Server("mixed", persistent=True) as srv:
+        sid = srv.add_snapshot(models[0])
+        first = [srv.submit_image(sid, **kw) for kw in kws]
+        time.sleep(0.002)  # the image-only launch has started (it gathers work for 0.2 ms)
+        sh = srv.submit_shadow(0.6, 40.0, ang, max_steps=1500)
+        later = [srv.submit_image(sid, **kw) for kw in kws]
+        for rid, ref in zip(first + later, refs + refs):
+            np.testing.assert_array_equal(srv.result(rid, timeout=120), ref)
+        np.testing.assert_array_equal(srv.result(sh, timeout=120), ref_sh)
+        time.sleep(0.1)
+        st = srv.stats()
+    print(f"[serving] image-only then bisection: {st['iterations']} launches")
+    assert st["completed"] == 7 and st["in_flight"] == 0
+    assert st["iterations"] >= 2
+
+
 @pytest.mark.parametrize("persistent", [False, True])
 def test_budget_back_pressure_and_errors(models, persistent):
     res = 48
