@@ -141,6 +141,11 @@ def test_octree_1024_mixed_vs_plain_flux(octree, precision):
     err = abs(mixed.sum() - plain.sum()) / plain.sum()
     print(f"[render] 1024^2 octree {precision}: flux rel err {err:.3e}")
     assert err < 1e-3
+    # the classic form's tiles from a counter against one wave per tile
+    p = images.render_params(0.94, camera_inclination=17, resolution=res, precision=precision)
+    assert _form_of(octree, p) == "classic"
+    p.flags |= MK_RENDER_STATIC_TILES
+    np.testing.assert_array_equal(images.render(octree.device_snapshot, p), mixed)
 
 
 @pytest.mark.parametrize("res,nsteps", [(13, 10000), (37, 10000), (24, 300), (9, 1)])
