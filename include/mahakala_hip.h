@@ -260,10 +260,14 @@ typedef struct mk_server_config {
  * publishes them, run each tile to the end and stay resident while work keeps
  * coming; the launch ends once the server has been idle for 20 ms (or, after
  * 1 s, whenever nothing is in flight) and the next request relaunches it.
- * Work, descriptors and outputs pass through pinned host memory, and
- * hbm_budget (default 1 GiB here) bounds the pinned output staging.  The
- * results are the same bits as the relaunched form's and mk_render's.
- * Statistics of this form are updated when a launch ends. */
+ * A launch with no bisection to serve runs a kernel without the bisection
+ * path, at three waves per SIMD (mixed, default lanes) when it starts with a
+ * deep backlog; a launch starting from idle first gathers work for 0.2 ms,
+ * and a bisection arriving during an image-only launch waits until that
+ * launch has drained.  Work, descriptors and outputs pass through pinned
+ * host memory, and hbm_budget (default 1 GiB here) bounds the pinned output
+ * staging.  The results are the same bits as the relaunched form's and
+ * mk_render's.  Statistics of this form are updated when a launch ends. */
 #define MK_SERVER_PERSISTENT 1
 
 typedef struct mk_image_request {
