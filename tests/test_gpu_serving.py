@@ -152,6 +152,42 @@ def test_persistent_bisection_arriving_during_an_image_only_launch(models):
 
 
 @pytest.mark.parametrize("persistent", [False, True])
+def test_concurrent_submitters(models, persistent):
+    """Four host threads submit, wait and read statistics at once (the C
+    calls release the GIL): every image equals mk_render's, and the counts
+    add up."""
+    import threading
+    specs = [(32 + 16 * (k % 3), 0.5 - 0.2 * (k % 4), 20.0 + 15 * (k % 5), (86e9, 230e9, 345e9)[k % 3], k % 2)
+             for k in range(20)]
+    refs = {k: _render(models[s], res, spin, inc, nu, "mixed") for k, (res, spin, inc, nu, s) in enumerate(specs)}
+    got, errors = {}, []
+    with Server("mixed", persistent=persistent) as srv:
+        sids = [srv.add_snapshot(m) for m in models]
+
+        def client(ks):
+            try:
+                for k in ks:
+                    res, spin, inc, nu, s = specs[k]
+                    rid = srv.submit_image(sids[s], **_img_kw(res, spin, inc, nu))
+                    srv.stats()
+                    got[k] = srv.result(rid, timeout=120)
+            except Exception as e:  # reported below, on the main thread
+                errors.append(repr(e))
+
+        threads = [threading.Thread(target=client, args=(range(t, 20, 4),)) for t in range(4)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join(timeout=300)
+        st = srv.stats()
+    assert not errors, errors
+    assert sorted(got) == list(range(20))
+    for k in range(20):
+        np.testing.assert_array_equal(got[k], refs[k])
+    assert st["submitted"] == 20 and st["completed"] == 20 and st["in_flight"] == 0
+
+
+@pytest.mark.parametrize("persistent", [False, True])
 def test_budget_back_pressure_and_errors(models, persistent):
     res = 48
     # one request: its output (256-byte aligned already) and its tile order
