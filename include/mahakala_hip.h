@@ -111,7 +111,12 @@ int mk_integrate_history(const double* d_s0, long n, int N, double div, double t
  * (radius, angle) camera point; host in, host out. */
 int mk_select_photons(double inclination_deg, const double* h_angle, const double* h_radius, long n, double bhspin,
                       double distance, int max_steps, double* h_out);
-/* find_shadow_bisection_angles  (geodesics.py:405-435): host in, host out. */
+/* find_shadow_bisection_angles  (geodesics.py:405-435): host in, host out.
+ * The radii are the sequential bisection's, bit for bit.  The schedule uses
+ * the critical curve of a distant camera as a prediction: up to 4096 angles
+ * the first round integrates every midpoint within a band of up to
+ * MK_BISECT_BAND (environment, M; default 0.05, 0 = off) about the
+ * predicted radius, so most angles finish in one round. */
 int mk_shadow_bisect(double bhspin, double inclination_deg, const double* h_angles, long n, int max_steps,
                      double error_allowed, int max_it, double distance, double* h_radii);
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -304,6 +305,176 @@ __global__ void __launch_bounds__(1024) k_bisect_tree(double a, double ci, doubl
   if (j == 0 && p < n) radii[p] = br[2 * g];
 }
 
+// Bracket indices [b0, b1] at level l (after l halvings the brackets are
+// 9.5 / 2^l wide from 0.5, index bits = the halvings' outcomes, 1 = fell)
+// that meet the band [rho - e, rho + e]; empty without a prediction.  Used
+// only to choose which midpoints to integrate: a wrong choice costs time,
+// never a result.
+__device__ __forceinline__ long long band_range(double rho, double e, int l, long long& b0) {
+  if (!(rho == rho)) {
+    b0 = 0;
+    return 0;
+  }
+  const long long nb = 1ll << l;
+  const double w = 9.5 / (double)nb;
+  b0 = (long long)__builtin_floor((rho - e - 0.5) / w);
+  long long b1 = (long long)__builtin_floor((rho + e - 0.5) / w);
+  if (b0 < 0) b0 = 0;
+  if (b1 > nb - 1) b1 = nb - 1;
+  return b1 >= b0 ? b1 - b0 + 1 : 0;
+}
+constexpr int kBandMaxLevels = 48;
+
+// Midpoints in the band over levels [0, L).
+__device__ __forceinline__ long long band_nodes(double rho, double e, int L) {
+  long long t = 0, b0;
+  for (int l = 0; l < L; ++l) t += band_range(rho, e, l, b0);
+  return t;
+}
+
+// The bisection in fewer rounds, guided by a predicted radius per angle
+// (radii[p] on entry, NaN for none): each angle's B lanes first integrate, in
+// one round, every midpoint whose bracket meets a band about the prediction:
+// the widest band of half-width up to e whose midpoints of every level fit
+// in B lanes with one angle per block (several angles per block: the band
+// of half-width e over as many levels as fit).
+// One lane then replays the halvings from the classifications while the path
+// stays in the band; the halvings left (path out of the band, or levels
+// beyond it) run in tree rounds as in k_bisect_tree.  Every midpoint is
+// computed along its path with the reference's (outer - inner) / 2 + inner
+// and the replay uses only classifications of the midpoints the sequential
+// bisection visits, so the result is its result, bit for bit, whatever the
+// prediction; the prediction decides only how many rounds it takes.  G
+// angles per block.
+template <int B, int G>
+__global__ void __launch_bounds__(B * G) k_bisect_band(double a, double ci, double si, double d, const double* cang,
+                                                       const double* sang, long n, int N, int iters, double rEH,
+                                                       double e, double* radii) {
+  constexpr int KB = B >= 255 ? 8 : B >= 127 ? 7 : B >= 63 ? 6 : B >= 31 ? 5 : 4;  // 2^KB - 1 <= B
+  __shared__ double br[2 * G];
+  __shared__ int cls[B * G], resolved[G];
+  const int g = threadIdx.x / B, j = threadIdx.x % B;
+  const long p = (long)blockIdx.x * G + g;
+  const bool valid = p < n;
+  const double rho = valid ? radii[p] : __builtin_nan("");
+  const int lmax = iters < kBandMaxLevels ? iters : kBandMaxLevels;
+  // one angle per block: narrow the band until every level fits (several
+  // angles per block keep the band and fewer levels: their lanes are few)
+  if (G == 1 && valid && band_nodes(rho, e, lmax) > B) {
+    double lo = 0.0, hi = e;
+    for (int it = 0; it < 30; ++it) {
+      const double m = 0.5 * (lo + hi);
+      if (band_nodes(rho, m, lmax) <= B) lo = m;
+      else hi = m;
+    }
+    e = lo;
+  }
+  // this lane's midpoint in the band: level ml, bracket mb
+  int L = 0, ml = -1;
+  long long mb = 0;
+  if (valid) {
+    long long total = 0;
+    for (int l = 0; l < lmax; ++l) {
+      long long b0;
+      const long long cnt = band_range(rho, e, l, b0);
+      if (cnt == 0 || total + cnt > B) break;
+      if (j >= total && j < total + cnt) {
+        ml = l;
+        mb = b0 + (j - total);
+      }
+      total += cnt;
+      L = l + 1;
+    }
+  }
+  int c = 0;
+  if (ml >= 0) {
+    double lo = 0.5, hi = 10.0, mid = (hi - lo) / 2 + lo;
+    for (int k = ml - 1; k >= 0; --k) {
+      if ((mb >> k) & 1) lo = mid;
+      else hi = mid;
+      mid = (hi - lo) / 2 + lo;
+    }
+    double sx[4], sv[4], st[8];
+    camera_ray(cang[p] * mid, sang[p] * mid, ci, si, d, sx, sv);
+    nullify(sx, sv, a, st);
+    WalkOut o;
+    walk_ray(st, N, 40.0, 1e-2, a, rEH, o);
+    c = o.sel_r < 100.0 ? 1 : 0;
+  }
+  cls[threadIdx.x] = c;
+  __syncthreads();
+  if (j == 0) {
+    double lo = 0.5, hi = 10.0;
+    long long b = 0, off = 0;
+    int l = 0;
+    for (; l < L; ++l) {
+      long long b0;
+      const long long cnt = band_range(rho, e, l, b0);
+      if (b < b0 || b >= b0 + cnt) break;  // the path left the band
+      const double mid = (hi - lo) / 2 + lo;
+      if (cls[g * B + (int)(off + b - b0)]) {
+        lo = mid;
+        b = 2 * b + 1;
+      } else {
+        hi = mid;
+        b = 2 * b;
+      }
+      off += cnt;
+    }
+    br[2 * g] = lo;
+    br[2 * g + 1] = hi;
+    resolved[g] = valid ? l : iters;
+  }
+  __syncthreads();
+  // the halvings left, KB or fewer per round (spread evenly over the rounds);
+  // every lane reads its angle's state after the barrier that follows its
+  // update
+  while (true) {
+    const int rem = iters - resolved[g];
+    if (!__syncthreads_or(rem > 0)) break;
+    const int rounds = (rem + KB - 1) / KB;
+    const int kr = rem > 0 ? (rem + rounds - 1) / rounds : 0;
+    c = 0;
+    if (j < (1 << kr) - 1) {
+      const int m = j + 1, depth = 31 - __builtin_clz(m);
+      double lo = br[2 * g], hi = br[2 * g + 1];
+      double mid = (hi - lo) / 2 + lo;
+      for (int k = depth - 1; k >= 0; --k) {
+        if ((m >> k) & 1) lo = mid;
+        else hi = mid;
+        mid = (hi - lo) / 2 + lo;
+      }
+      double sx[4], sv[4], st[8];
+      camera_ray(cang[p] * mid, sang[p] * mid, ci, si, d, sx, sv);
+      nullify(sx, sv, a, st);
+      WalkOut o;
+      walk_ray(st, N, 40.0, 1e-2, a, rEH, o);
+      c = o.sel_r < 100.0 ? 1 : 0;
+    }
+    cls[threadIdx.x] = c;
+    __syncthreads();
+    if (j == 0 && kr > 0) {
+      double lo = br[2 * g], hi = br[2 * g + 1];
+      int node = 0;
+      for (int l = 0; l < kr; ++l) {
+        const double mid = (hi - lo) / 2 + lo;
+        if (cls[g * B + node]) {
+          lo = mid;
+          node = 2 * node + 2;
+        } else {
+          hi = mid;
+          node = 2 * node + 1;
+        }
+      }
+      br[2 * g] = lo;
+      br[2 * g + 1] = hi;
+      resolved[g] += kr;
+    }
+    __syncthreads();
+  }
+  if (j == 0 && valid) radii[p] = br[2 * g];
+}
+
 __global__ void k_synchrotron(const double* Ne, const double* Th, const double* B, const double* pitch,
                               const double* nu, long n, int invariant, double rescale, double* j, double* al) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -397,8 +568,9 @@ double horizon(double a) { return 1.0 + std::sqrt(1.0 - a * a); }
 using namespace mk;
 
 // ------------------------------------------------ allocation-free staging
-// mk_shadow_bisect_noalloc keeps its staging (cos and sin of the angles in,
-// radii out) in this fixed array instead of allocating: each GPU holds its
+// mk_shadow_bisect_noalloc keeps its staging (cos and sin of the angles in;
+// the predicted radii in and the radii out, one slot) in this fixed array
+// instead of allocating: each GPU holds its
 // own copy (1.5 MiB) until the process ends.  Host code reaches it only
 // through the per-device address from hipGetSymbolAddress; the array's name
 // in host code is not a device address.  The host side of the staging is a
@@ -410,7 +582,7 @@ namespace {
 constexpr int kMaxDevices = 64;
 std::mutex g_noalloc_mu;
 double* g_noalloc_dev[kMaxDevices] = {};
-double g_noalloc_host[2 * MK_NOALLOC_MAX_ANGLES];
+double g_noalloc_host[3 * MK_NOALLOC_MAX_ANGLES];  // cos, sin, predicted radii
 
 // The current device's copy of `scratch`, looked up once per device (called
 // with g_noalloc_mu held).
@@ -710,12 +882,149 @@ static int bisect_iterations(double error_allowed, int max_it) {
   return iters;
 }
 
+// Half-width (M) of the band about the predicted shadow radius that the
+// bisection integrates in its first round: MK_BISECT_BAND in the environment,
+// default 0.05 M (the predictions below are within 0.005 M of the bisection's
+// result at moderate spins and 0.03 M at a = 0.99, i = 85 deg); 0 turns the
+// band off.
+static double bisect_band() {
+  const char* v = std::getenv("MK_BISECT_BAND");
+  if (!v || !*v) return 0.05;
+  const double e = std::atof(v);
+  return e > 0.0 && e < 10.0 ? e : 0.0;
+}
+
+// Predicted shadow radius along each image direction: the distance from the
+// image origin to the critical curve of a distant camera (Bardeen 1973;
+// image x is alpha, image y is beta in this camera's convention), from a
+// polar table of the curve about the origin; NaN where the table has no
+// value.  Only the bisection's schedule depends on it.
+static void predicted_shadow_radii(double a, double inc_deg, const double* c, const double* s, long n,
+                                   double* rho) {
+  constexpr int kPts = 1024, kPolar = 2048;
+  const double th = std::min(std::max(inc_deg, 1.0), 179.0) * M_PI / 180.0;
+  std::vector<double> px, py;
+  if (std::fabs(a) < 1e-3) {
+    for (int q = 0; q < 2 * kPts; ++q) {
+      px.push_back(std::sqrt(27.0) * std::cos(M_PI * q / kPts));
+      py.push_back(std::sqrt(27.0) * std::sin(M_PI * q / kPts));
+    }
+  } else {
+    const double aa = std::max(-0.9999, std::min(0.9999, a));
+    const double r1 = 2.0 * (1.0 + std::cos(2.0 / 3.0 * std::acos(-std::fabs(aa))));
+    const double r2 = 2.0 * (1.0 + std::cos(2.0 / 3.0 * std::acos(std::fabs(aa))));
+    const double ct = std::cos(th) / std::sin(th), c2 = std::cos(th) * std::cos(th);
+    auto xi_of = [&](double r) { return (r * r * (3.0 - r) - aa * aa * (r + 1.0)) / (aa * (r - 1.0)); };
+    auto beta2 = [&](double r) {
+      const double xi = xi_of(r);
+      const double eta = r * r * r * (4.0 * aa * aa - r * (r - 3.0) * (r - 3.0)) / (aa * aa * (r - 1.0) * (r - 1.0));
+      return eta + aa * aa * c2 - xi * xi * ct * ct;
+    };
+    // the orbit radii the camera sees (beta^2 >= 0: one interval), found on a
+    // fine scan and refined by bisection at both ends
+    const double rmin = std::min(r1, r2), rmax = std::max(r1, r2);
+    constexpr int kScan = 8192;
+    int q0 = -1, q1 = -1;
+    for (int q = 0; q <= kScan; ++q) {
+      const double r = rmin + (rmax - rmin) * q / kScan;
+      if (std::fabs(r - 1.0) > 1e-9 && beta2(r) >= 0.0) {
+        if (q0 < 0) q0 = q;
+        q1 = q;
+      }
+    }
+    std::vector<double> al, be;
+    if (q0 >= 0 && q1 > q0) {
+      auto edge = [&](int in, int out) {  // the sign change of beta^2 between two scan points
+        double u = rmin + (rmax - rmin) * in / kScan, v = rmin + (rmax - rmin) * out / kScan;
+        for (int it = 0; it < 60; ++it) {
+          const double w = 0.5 * (u + v);
+          if (beta2(w) >= 0.0) u = w;
+          else v = w;
+        }
+        return u;
+      };
+      const double lo = q0 > 0 ? edge(q0, q0 - 1) : rmin, hi = q1 < kScan ? edge(q1, q1 + 1) : rmax;
+      // beta ~ sqrt(r - lo) at the ends: samples clustered there (r - lo ~ u^2)
+      // are spread evenly along the curve
+      for (int q = 0; q <= kPts; ++q) {
+        const double r = lo + (hi - lo) * 0.5 * (1.0 - std::cos(M_PI * q / kPts));
+        if (std::fabs(r - 1.0) < 1e-9) continue;
+        al.push_back(-xi_of(r) / std::sin(th));
+        be.push_back(std::sqrt(std::max(beta2(r), 0.0)));
+      }
+    }
+    // closed curve: upper branch, then the lower one back
+    for (size_t q = 0; q < al.size(); ++q) {
+      px.push_back(al[q]);
+      py.push_back(be[q]);
+    }
+    for (size_t q = al.size(); q-- > 0;) {
+      px.push_back(al[q]);
+      py.push_back(-be[q]);
+    }
+  }
+  std::vector<double> R(kPolar, std::nan(""));
+  const size_t m = px.size();
+  if (m >= 8) {
+    // each edge's intersections with the rays at the bin centres it spans
+    for (size_t q = 0; q < m; ++q) {
+      const double ax = px[q], ay = py[q], bx = px[(q + 1) % m], by = py[(q + 1) % m];
+      const double fa = std::atan2(ay, ax), fb = std::atan2(by, bx);
+      double df = fb - fa;
+      if (df > M_PI) df -= 2.0 * M_PI;
+      if (df < -M_PI) df += 2.0 * M_PI;
+      const double f0 = std::min(fa, fa + df), f1 = std::max(fa, fa + df);
+      const double ex = bx - ax, ey = by - ay;
+      for (long k = (long)std::ceil((f0 + M_PI) / (2.0 * M_PI) * kPolar - 0.5);
+           k <= (long)std::floor((f1 + M_PI) / (2.0 * M_PI) * kPolar - 0.5); ++k) {
+        const double f = -M_PI + (k + 0.5) * 2.0 * M_PI / kPolar;
+        const double dx = std::cos(f), dy = std::sin(f), den = dx * ey - dy * ex;
+        if (den == 0.0) continue;
+        const double t = (ax * ey - ay * ex) / den;
+        const int kb = (int)(((k % kPolar) + kPolar) % kPolar);
+        if (t > 0.0 && !(R[kb] >= t)) R[kb] = t;
+      }
+    }
+  }
+  for (long p = 0; p < n; ++p) {
+    const double u = (std::atan2(s[p], c[p]) + M_PI) / (2.0 * M_PI) * kPolar - 0.5;
+    const double fl = std::floor(u);
+    const int k0 = (int)((((long)fl % kPolar) + kPolar) % kPolar), k1 = (k0 + 1) % kPolar;
+    const double w = u - fl;
+    rho[p] = (1.0 - w) * R[k0] + w * R[k1];  // NaN when either bin is empty
+  }
+}
+
 // Launches the bisection of n angles (cos and sin in dc, ds; device) into dr
-// on the null stream.
+// on the null stream.  h_rho (n host doubles, may be null): the predicted
+// radii, staged through dr for the band form.
 static hipError_t launch_bisect(double a, double inc, const double* dc, const double* ds, long n, int max_steps,
-                                int iters, double dist, double* dr) {
+                                int iters, double dist, double* dr, const double* h_rho) {
   double ci, si;
   cam_angles(inc, &ci, &si);
+  // The band form.  The FP64 walk slows with the
+  // waves resident on a CU, so up to 1024 angles each angle is a block of
+  // its own: one wave (the band then fits about 0.012 M about the
+  // prediction, which is within 0.007 M of the result for |a| <= 0.9) or, for
+  // higher spins, where the prediction drifts further (0.02 M at 0.97, more
+  // at high inclination near a = 1), two waves (about 0.03 M).  From 2049
+  // to 4096 angles, blocks of 16 angles of 16 lanes (4096 angles: 10.8
+  // against 12.8 ms in tree rounds; at 2048, 10.9 against 10.0, so the tree
+  // rounds keep 1025-2048).
+  const double e = bisect_band();
+  if (h_rho && e > 0.0 && iters > 1 && (n <= 1024 || (n > 2048 && n <= 4096))) {
+    hipError_t err = hipMemcpy(dr, h_rho, n * sizeof(double), hipMemcpyHostToDevice);
+    if (err != hipSuccess) return err;
+    const double rEH = horizon(a);
+    if (n <= 1024 && std::fabs(a) <= 0.9)
+      k_bisect_band<64, 1><<<(unsigned)n, 64>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, rEH, e, dr);
+    else if (n <= 1024)
+      k_bisect_band<128, 1><<<(unsigned)n, 128>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, rEH, e, dr);
+    else
+      k_bisect_band<16, 16><<<(unsigned)((n + 15) / 16), 256>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, rEH,
+                                                                 e, dr);
+    return hipGetLastError();
+  }
   // K halvings per round with 2^K lanes per angle: the fewest rounds with at
   // most about 65 536 lanes at once, then the halvings spread evenly over
   // those rounds (fewer lanes for the same number of rounds; the slowest
@@ -749,18 +1058,19 @@ int mk_shadow_bisect(double a, double inc, const double* h_angles, long n, int m
   if (!n) return MK_OK;
   // every angle shares one bracket width, so the count is fixed up front
   const int iters = bisect_iterations(error_allowed, max_it);
-  std::vector<double> hc(n), hs(n);
+  std::vector<double> hc(n), hs(n), hr(n);
   for (long p = 0; p < n; ++p) {
     hc[p] = std::cos(h_angles[p]);
     hs[p] = std::sin(h_angles[p]);
   }
+  predicted_shadow_radii(a, inc, hc.data(), hs.data(), n, hr.data());
   double *dc = nullptr, *ds = nullptr, *dr = nullptr;
   hipError_t e = hipMalloc(&dc, n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&ds, n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&dr, n * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(dc, hc.data(), n * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(ds, hs.data(), n * sizeof(double), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_bisect(a, inc, dc, ds, n, max_steps, iters, dist, dr);
+  if (e == hipSuccess) e = launch_bisect(a, inc, dc, ds, n, max_steps, iters, dist, dr, hr.data());
   if (e == hipSuccess) e = hipMemcpy(h_radii, dr, n * sizeof(double), hipMemcpyDeviceToHost);
   (void)hipFree(dc);
   (void)hipFree(ds);
@@ -785,10 +1095,12 @@ int mk_shadow_bisect_noalloc(double a, double inc, const double* h_angles, long 
   if (rc) return rc;
   double* hc = g_noalloc_host;
   double* hs = g_noalloc_host + MK_NOALLOC_MAX_ANGLES;
+  double* hr = g_noalloc_host + 2 * MK_NOALLOC_MAX_ANGLES;
   for (long p = 0; p < n; ++p) {
     hc[p] = std::cos(h_angles[p]);
     hs[p] = std::sin(h_angles[p]);
   }
+  predicted_shadow_radii(a, inc, hc, hs, n, hr);
   // scratch: cos[0, MAX), sin[MAX, 2 MAX), radii[2 MAX, 3 MAX); every value
   // a launch reads (p < n) is written first, so nothing needs clearing
   double* dc = scr;
@@ -796,7 +1108,7 @@ int mk_shadow_bisect_noalloc(double a, double inc, const double* h_angles, long 
   double* dr = scr + 2 * MK_NOALLOC_MAX_ANGLES;
   hipError_t e = hipMemcpy(dc, hc, n * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(ds, hs, n * sizeof(double), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_bisect(a, inc, dc, ds, n, max_steps, iters, dist, dr);
+  if (e == hipSuccess) e = launch_bisect(a, inc, dc, ds, n, max_steps, iters, dist, dr, hr);
   if (e == hipSuccess) e = hipMemcpy(h_radii, dr, n * sizeof(double), hipMemcpyDeviceToHost);
   if (e != hipSuccess) {
     set_error("mk_shadow_bisect_noalloc: %s", hipGetErrorString(e));

@@ -122,17 +122,37 @@ def test_shadow_64_angles_and_select():
     np.testing.assert_allclose(sel, f["select_r"], rtol=1e-8)
 
 
-def test_shadow_rounds_of_several_halvings_equal_the_sequential_bisection():
-    """mk_shadow_bisect runs K halvings per round (2^K - 1 midpoints per angle
-    integrated at once, K from the angle count: 7 for 64 angles, 5 for 1024,
-    4 for 4096) and one lane per angle from 32768 angles on; the radii are the
-    same bits either way."""
+def test_shadow_rounds_of_several_halvings_equal_the_sequential_bisection(monkeypatch):
+    """Up to 4096 angles, mk_shadow_bisect first integrates, in one round, every
+    midpoint of the first halvings whose bracket meets a band about the shadow
+    radius predicted from the critical curve (64 lanes per angle up to 1024
+    angles at |a| <= 0.9, 128 above; 16 from 2049 to 4096 angles), replays
+    the halvings while the path stays in the band, and runs the rest in tree
+    rounds (2^K - 1 midpoints of the next K halvings at once); from 1025 to
+    2048 and above 4096 angles, tree rounds only (K from the angle count), and
+    one lane per angle from 32768 angles on.  The radii are the same bits either way: with the
+    default band, with a band of 1e-4 M (the path leaves it at most angles, so
+    the tree rounds finish them), and with the band off (MK_BISECT_BAND=0,
+    tree rounds from the first halving)."""
     ang = np.arange(4096) / 4096 * 2 * np.pi
     seq = ma.find_shadow_bisection_angles(0.9, 60, np.tile(ang, 16))  # 65536 angles: one lane each
     np.testing.assert_array_equal(seq[:4096], seq[4096:8192])
-    for sub in (ang[::64], ang[::4], ang):  # 64, 1024, 4096 angles
-        got = ma.find_shadow_bisection_angles(0.9, 60, sub)
-        np.testing.assert_array_equal(got, seq[:4096][np.isin(ang, sub)])
+    for band in (None, "1e-4", "0"):
+        if band is None:
+            monkeypatch.delenv("MK_BISECT_BAND", raising=False)
+        else:
+            monkeypatch.setenv("MK_BISECT_BAND", band)
+        for sub in (ang[::64], ang[::16], ang[::4], ang[::2], ang):  # 64, 256, 1024, 2048, 4096 angles
+            got = ma.find_shadow_bisection_angles(0.9, 60, sub)
+            np.testing.assert_array_equal(got, seq[:4096][np.isin(ang, sub)],
+                                          err_msg=f"band {band}, {sub.size} angles")
+    monkeypatch.delenv("MK_BISECT_BAND", raising=False)
+    # near-extremal, nearly edge-on (the prediction is 0.03 M off on the flat
+    # side) and nearly face-on, against the C restatement
+    for a, inc in ((0.99, 85.0), (0.998, 2.0), (-0.95, 70.0)):
+        sub = ang[::128]
+        np.testing.assert_array_equal(ma.find_shadow_bisection_angles(a, inc, sub),
+                                      c_oracle.find_shadow_bisection_angles(a, inc, sub), err_msg=f"a={a} i={inc}")
 
 
 def test_schwarzschild_shadow():
