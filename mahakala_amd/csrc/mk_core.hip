@@ -26,6 +26,10 @@ void set_error(const char* fmt, ...) {
 }
 
 constexpr int kBlock = 256;
+// Per-ray walks (integrate, select) launch one wave per workgroup: a launch of
+// a few thousand rays then spreads over as many CUs as it has waves, and the
+// FP64 walk runs faster with fewer waves resident on a CU.
+constexpr int kChainBlock = 64;
 
 // ------------------------------------------------------------------ kernels
 __global__ void k_camera_grid(double a, double ci, double si, double d, double lo, double step, int res,
@@ -802,7 +806,7 @@ int mk_integrate_final(const double* d_s0, long n, int N, double div, double tol
                        double* d_last, double* d_sel, void* s) {
   MK_REQUIRE(n >= 0 && N >= 1 && d_s0 && div != 0.0, "mk_integrate_final: bad arguments");
   if (!n) return MK_OK;
-  k_integrate_final<<<grid_for(n, kBlock), kBlock, 0, as_stream(s)>>>(d_s0, n, N, div, tol, a, horizon(a), d_K,
+  k_integrate_final<<<grid_for(n, kChainBlock), kChainBlock, 0, as_stream(s)>>>(d_s0, n, N, div, tol, a, horizon(a), d_K,
                                                                       d_last, d_sel);
   MK_LAUNCH_CHECK();
   return MK_OK;
@@ -839,7 +843,7 @@ int mk_integrate_history(const double* d_s0, long n, int N, double div, double t
                          double* d_dt, void* s) {
   MK_REQUIRE(n >= 0 && T >= 0 && T <= N && d_s0 && d_S && d_dt, "mk_integrate_history: bad arguments");
   if (!n || !T) return MK_OK;
-  k_integrate_history<<<grid_for(n, kBlock), kBlock, 0, as_stream(s)>>>(d_s0, n, div, tol, a, horizon(a), T, d_S,
+  k_integrate_history<<<grid_for(n, kChainBlock), kChainBlock, 0, as_stream(s)>>>(d_s0, n, div, tol, a, horizon(a), T, d_S,
                                                                         d_dt);
   MK_LAUNCH_CHECK();
   return MK_OK;
@@ -856,7 +860,7 @@ int mk_select_photons(double inc, const double* h_angle, const double* h_radius,
   hipError_t e = hipSuccess;
   if (rc == MK_OK) e = hipMalloc(&dout, n * sizeof(double));
   if (rc == MK_OK && e == hipSuccess) {
-    k_select<<<grid_for(n, kBlock), kBlock>>>(a, ci, si, dist, dx, dy, n, max_steps, horizon(a), dout);
+    k_select<<<grid_for(n, kChainBlock), kChainBlock>>>(a, ci, si, dist, dx, dy, n, max_steps, horizon(a), dout);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpy(h_out, dout, n * sizeof(double), hipMemcpyDeviceToHost);
   }
