@@ -155,6 +155,26 @@ def test_shadow_rounds_of_several_halvings_equal_the_sequential_bisection(monkey
                                       c_oracle.find_shadow_bisection_angles(a, inc, sub), err_msg=f"a={a} i={inc}")
 
 
+@pytest.mark.parametrize("dist,err,max_it", [(200.0, 1e-3, 40), (1000.0, 1e-3, 5), (1000.0, 0.1, 40),
+                                              (1000.0, 1e-6, 40)])
+def test_shadow_bisection_other_distances_and_depths(dist, err, max_it):
+    """mk_shadow_bisect at a closer camera (the distant-camera prediction is
+    further off) and at 5, 7 and 24 halvings (the band over every level, and a
+    band that fits only a narrow path before the tree rounds): the C
+    restatement's radii, bit for bit, for 64 angles (one and two waves per
+    angle) and 2048 angles (tree rounds only)."""
+    from mahakala_amd import _lib
+    lib = _lib.lib()
+    for n, a, inc in ((64, 0.9, 60.0), (64, 0.95, 80.0), (2048, 0.5, 30.0)):
+        ang = np.arange(n) / n * 2 * np.pi
+        out = np.empty(n)
+        _lib.check(lib.mk_shadow_bisect(a, inc, ang.ctypes.data, n, 2000, err, max_it, dist, out.ctypes.data))
+        ref = c_oracle.find_shadow_bisection_angles(a, inc, ang, 2000, err, max_it, distance=dist)
+        print(f"[shadow] a={a} i={inc} {n} angles, distance {dist}, {c_oracle.bisect_iterations(err, max_it)} "
+              f"halvings: identical {np.array_equal(out, ref)}")
+        np.testing.assert_array_equal(out, ref, err_msg=f"a={a} i={inc} n={n}")
+
+
 def test_schwarzschild_shadow():
     r = ma.find_shadow_bisection_angles(0.0, 17.0, np.linspace(0, 6, 12))
     assert np.all(np.abs(r - np.sqrt(27)) < 9.5 / 2 ** 14 + 1e-3)
