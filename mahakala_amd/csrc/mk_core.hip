@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(1024) k_bisect_tree(double a, double ci, doubl
 // only to choose which midpoints to integrate: a wrong choice costs time,
 // never a result.
 __device__ __forceinline__ long long band_range(double rho, double e, int l, long long& b0) {
-  if (!(rho == rho)) {
+  if (!(rho > 0.0 && rho < 100.0)) {  // NaN, or nothing a bracket in [0.5, 10] could meet
     b0 = 0;
     return 0;
   }
