@@ -898,14 +898,22 @@ static double bisect_band() {
   return e > 0.0 && e < 10.0 ? e : 0.0;
 }
 
-// Predicted shadow radius along each image direction: the distance from the
-// image origin to the critical curve of a distant camera (Bardeen 1973;
-// image x is alpha, image y is beta in this camera's convention), from a
-// polar table of the curve about the origin; NaN where the table has no
-// value.  Only the bisection's schedule depends on it.
-static void predicted_shadow_radii(double a, double inc_deg, const double* c, const double* s, long n,
-                                   double* rho) {
-  constexpr int kPts = 1024, kPolar = 2048;
+// Critical curve of a distant camera (Bardeen 1973; image x is alpha, image y
+// is beta in this camera's convention) as a polar table about the image
+// origin: kPolar bins of the distance from the origin to the curve along the
+// bin centre's direction, NaN where the curve does not cross.
+constexpr int kPolar = 2048;
+static void critical_curve_about_origin(double a, double inc_deg, std::vector<double>& R) {
+  constexpr int kPts = 1024, kScan = 1024;
+  static const std::vector<double> dir = [] {  // cos, sin of the bin centres
+    std::vector<double> d(2 * kPolar);
+    for (int k = 0; k < kPolar; ++k) {
+      const double f = -M_PI + (k + 0.5) * 2.0 * M_PI / kPolar;
+      d[2 * k] = std::cos(f);
+      d[2 * k + 1] = std::sin(f);
+    }
+    return d;
+  }();
   const double th = std::min(std::max(inc_deg, 1.0), 179.0) * M_PI / 180.0;
   std::vector<double> px, py;
   if (std::fabs(a) < 1e-3) {
@@ -925,9 +933,8 @@ static void predicted_shadow_radii(double a, double inc_deg, const double* c, co
       return eta + aa * aa * c2 - xi * xi * ct * ct;
     };
     // the orbit radii the camera sees (beta^2 >= 0: one interval), found on a
-    // fine scan and refined by bisection at both ends
+    // scan and refined by bisection at both ends
     const double rmin = std::min(r1, r2), rmax = std::max(r1, r2);
-    constexpr int kScan = 8192;
     int q0 = -1, q1 = -1;
     for (int q = 0; q <= kScan; ++q) {
       const double r = rmin + (rmax - rmin) * q / kScan;
@@ -936,7 +943,6 @@ static void predicted_shadow_radii(double a, double inc_deg, const double* c, co
         q1 = q;
       }
     }
-    std::vector<double> al, be;
     if (q0 >= 0 && q1 > q0) {
       auto edge = [&](int in, int out) {  // the sign change of beta^2 between two scan points
         double u = rmin + (rmax - rmin) * in / kScan, v = rmin + (rmax - rmin) * out / kScan;
@@ -949,49 +955,74 @@ static void predicted_shadow_radii(double a, double inc_deg, const double* c, co
       };
       const double lo = q0 > 0 ? edge(q0, q0 - 1) : rmin, hi = q1 < kScan ? edge(q1, q1 + 1) : rmax;
       // beta ~ sqrt(r - lo) at the ends: samples clustered there (r - lo ~ u^2)
-      // are spread evenly along the curve
+      // are spread evenly along the curve; upper branch, then the lower one back
+      std::vector<double> al, be;
       for (int q = 0; q <= kPts; ++q) {
         const double r = lo + (hi - lo) * 0.5 * (1.0 - std::cos(M_PI * q / kPts));
         if (std::fabs(r - 1.0) < 1e-9) continue;
         al.push_back(-xi_of(r) / std::sin(th));
         be.push_back(std::sqrt(std::max(beta2(r), 0.0)));
       }
-    }
-    // closed curve: upper branch, then the lower one back
-    for (size_t q = 0; q < al.size(); ++q) {
-      px.push_back(al[q]);
-      py.push_back(be[q]);
-    }
-    for (size_t q = al.size(); q-- > 0;) {
-      px.push_back(al[q]);
-      py.push_back(-be[q]);
-    }
-  }
-  std::vector<double> R(kPolar, std::nan(""));
-  const size_t m = px.size();
-  if (m >= 8) {
-    // each edge's intersections with the rays at the bin centres it spans
-    for (size_t q = 0; q < m; ++q) {
-      const double ax = px[q], ay = py[q], bx = px[(q + 1) % m], by = py[(q + 1) % m];
-      const double fa = std::atan2(ay, ax), fb = std::atan2(by, bx);
-      double df = fb - fa;
-      if (df > M_PI) df -= 2.0 * M_PI;
-      if (df < -M_PI) df += 2.0 * M_PI;
-      const double f0 = std::min(fa, fa + df), f1 = std::max(fa, fa + df);
-      const double ex = bx - ax, ey = by - ay;
-      for (long k = (long)std::ceil((f0 + M_PI) / (2.0 * M_PI) * kPolar - 0.5);
-           k <= (long)std::floor((f1 + M_PI) / (2.0 * M_PI) * kPolar - 0.5); ++k) {
-        const double f = -M_PI + (k + 0.5) * 2.0 * M_PI / kPolar;
-        const double dx = std::cos(f), dy = std::sin(f), den = dx * ey - dy * ex;
-        if (den == 0.0) continue;
-        const double t = (ax * ey - ay * ex) / den;
-        const int kb = (int)(((k % kPolar) + kPolar) % kPolar);
-        if (t > 0.0 && !(R[kb] >= t)) R[kb] = t;
+      for (size_t q = 0; q < al.size(); ++q) {
+        px.push_back(al[q]);
+        py.push_back(be[q]);
+      }
+      for (size_t q = al.size(); q-- > 0;) {
+        px.push_back(al[q]);
+        py.push_back(-be[q]);
       }
     }
   }
+  R.assign(kPolar, std::nan(""));
+  const size_t m = px.size();
+  if (m < 8) return;
+  std::vector<double> ph(m);
+  for (size_t q = 0; q < m; ++q) ph[q] = std::atan2(py[q], px[q]);
+  // each edge's intersections with the rays at the bin centres it spans
+  for (size_t q = 0; q < m; ++q) {
+    const size_t u = (q + 1) % m;
+    const double ax = px[q], ay = py[q], ex = px[u] - ax, ey = py[u] - ay;
+    double df = ph[u] - ph[q];
+    if (df > M_PI) df -= 2.0 * M_PI;
+    if (df < -M_PI) df += 2.0 * M_PI;
+    const double f0 = std::min(ph[q], ph[q] + df), f1 = std::max(ph[q], ph[q] + df);
+    for (long k = (long)std::ceil((f0 + M_PI) / (2.0 * M_PI) * kPolar - 0.5);
+         k <= (long)std::floor((f1 + M_PI) / (2.0 * M_PI) * kPolar - 0.5); ++k) {
+      const int kb = (int)(((k % kPolar) + kPolar) % kPolar);
+      const double dx = dir[2 * kb], dy = dir[2 * kb + 1], den = dx * ey - dy * ex;
+      if (den == 0.0) continue;
+      const double t = (ax * ey - ay * ex) / den;
+      if (t > 0.0 && !(R[kb] >= t)) R[kb] = t;
+    }
+  }
+}
+
+// Predicted shadow radius along each image direction (angle ang[p], the
+// image x axis at 0): the critical curve's table, interpolated between bin
+// centres; NaN where the table has no value.  Only the bisection's schedule depends on it.  The
+// tables of the last few (spin, inclination) pairs are kept.
+namespace {
+std::mutex g_curve_mu;
+std::vector<std::pair<std::pair<double, double>, std::vector<double>>> g_curves;  // most recent last
+constexpr size_t kCurvesKept = 8;
+}  // namespace
+
+static void predicted_shadow_radii(double a, double inc_deg, const double* ang, long n, double* rho) {
+  std::vector<double> R;
+  {
+    std::lock_guard<std::mutex> lk(g_curve_mu);
+    for (const auto& e : g_curves)
+      if (e.first.first == a && e.first.second == inc_deg) R = e.second;
+  }
+  if (R.empty()) {
+    critical_curve_about_origin(a, inc_deg, R);
+    std::lock_guard<std::mutex> lk(g_curve_mu);
+    if (g_curves.size() >= kCurvesKept) g_curves.erase(g_curves.begin());
+    g_curves.push_back({{a, inc_deg}, R});
+  }
   for (long p = 0; p < n; ++p) {
-    const double u = (std::atan2(s[p], c[p]) + M_PI) / (2.0 * M_PI) * kPolar - 0.5;
+    const double f = ang[p] - 2.0 * M_PI * std::floor((ang[p] + M_PI) / (2.0 * M_PI));  // in [-pi, pi)
+    const double u = (f + M_PI) / (2.0 * M_PI) * kPolar - 0.5;
     const double fl = std::floor(u);
     const int k0 = (int)((((long)fl % kPolar) + kPolar) % kPolar), k1 = (k0 + 1) % kPolar;
     const double w = u - fl;
@@ -1067,7 +1098,7 @@ int mk_shadow_bisect(double a, double inc, const double* h_angles, long n, int m
     hc[p] = std::cos(h_angles[p]);
     hs[p] = std::sin(h_angles[p]);
   }
-  predicted_shadow_radii(a, inc, hc.data(), hs.data(), n, hr.data());
+  predicted_shadow_radii(a, inc, h_angles, n, hr.data());
   double *dc = nullptr, *ds = nullptr, *dr = nullptr;
   hipError_t e = hipMalloc(&dc, n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&ds, n * sizeof(double));
@@ -1104,7 +1135,7 @@ int mk_shadow_bisect_noalloc(double a, double inc, const double* h_angles, long 
     hc[p] = std::cos(h_angles[p]);
     hs[p] = std::sin(h_angles[p]);
   }
-  predicted_shadow_radii(a, inc, hc, hs, n, hr);
+  predicted_shadow_radii(a, inc, h_angles, n, hr);
   // scratch: cos[0, MAX), sin[MAX, 2 MAX), radii[2 MAX, 3 MAX); every value
   // a launch reads (p < n) is written first, so nothing needs clearing
   double* dc = scr;
