@@ -577,6 +577,31 @@ def main():
                                            "over +-50 M with log-normal turbulence (sigma 0.5, seed 0)")
         print(f"[bench] octree 1024^2: " + ", ".join(f"{k} {v['ms_per_image']} ms" for k, v in oc.items()),
               file=sys.stderr, flush=True)
+        if not args.no_cpu:
+            # config 3 on the host cores at a reduced resolution: the same view
+            # at 256^2 through the C FP64 restatement, against the GPU's plain
+            # FP64 image of that size
+            from tests.helpers import c_snapshot_from_model
+            rc3 = 256
+            nth3 = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            cso = c_snapshot_from_model(mo)
+            t1 = time.perf_counter()
+            I3, _ = cso.render(0.94, 17.0, 1000.0, 20.0, rc3, 10000, 13. / 9, 40.0, 1e26, 6.2e9 * MSUN, 230e9,
+                               nthreads=nth3)
+            dt3 = time.perf_counter() - t1
+            p3 = images.render_params(0.94, camera_inclination=17.0, resolution=rc3, observing_frequency=230e9,
+                                      precision="plain64")
+            g3 = L.DeviceBuffer(8 * rc3 * rc3)
+            L.check(L.lib().mk_render(mo.device_snapshot.handle, p3, g3.ptr, None, None))
+            G3 = g3.to_array((rc3, rc3))
+            f3 = float(G3.sum())
+            out["cpu_baseline_config3"] = {
+                "value": round(rc3 * rc3 / dt3, 1), "unit": "rays/s", "cores": nth3, "kind": "port",
+                "sample": f"config 3's view at {rc3}x{rc3} instead of 1024x1024 (same camera, fov and octree), C FP64 "
+                          f"restatement with OpenMP ({dt3:.1f} s wall)",
+                "flux_rel_err_vs_gpu_plain": abs(float(I3.sum()) - f3) / f3}
+            print(f"[bench] config 3 on the CPU at {rc3}^2: {rc3 * rc3 / dt3:.0f} rays/s", file=sys.stderr, flush=True)
+            del cso, g3
         del mo, imgo
 
     if not args.no_shadow:
