@@ -743,6 +743,20 @@ def main():
                       f"restatement with the reference's dataflow, T = {S_np.shape[0]} rows ({dt_np_s:.1f} s)",
             "pixel_max_rel_err_vs_gpu_plain": float(np.max(np.abs(I_np - ref)[ref > 0] / ref[ref > 0]))}
         print(f"[bench] NumPy restatement: {idx.size / dt_np_s:.1f} rays/s", file=sys.stderr, flush=True)
+        if "config1_grid" in out:
+            # config 1 through the same NumPy dataflow, on 256 of its 4096 rays
+            # drawn at random (seed 0); their frozen rows against the GPU's
+            idx1 = np.sort(np.random.Generator(np.random.PCG64(0)).choice(n1, 256, replace=False))
+            t1 = time.perf_counter()
+            _, _, K_np = onp.geodesic_integrator(10000, s1[idx1], 40, tol1, a1, return_steps=True)
+            dt_np1 = time.perf_counter() - t1
+            out["cpu_baseline_numpy_config1"] = {
+                "value": round(idx1.size / dt_np1, 2), "unit": "rays/s", "cores": 1, "kind": "port",
+                "sample": f"{idx1.size} of config 1's 4096 rays drawn at random (seed 0), NumPy FP64 restatement "
+                          f"with the reference's dataflow ({dt_np1:.1f} s)",
+                "frozen_rows_equal_gpu": bool(np.array_equal(K_np, K1[idx1]))}
+            print(f"[bench] NumPy restatement, config 1: {idx1.size / dt_np1:.1f} rays/s", file=sys.stderr,
+                  flush=True)
     print(json.dumps(out), flush=True)
 
 
