@@ -365,7 +365,7 @@ def main():
                     help="only run the device copy kernel over MIB MiB 3 times (traffic counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported (run with --gpus 1)",
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in ``mahakala_amd`` may import, link or
 run anything in this directory; only ``tests/``, ``__graft_entry__.smoke()``
-and the ``cpu_baseline`` leg of ``bench.py`` use it, and only as the checker.
+and the ``cpu_baseline`` leg of ``bench.py`` use it, and only to check results
+or to time the CPU baseline.
 
 Two restatements live here:
 
