@@ -123,7 +123,8 @@ int mk_shadow_bisect(double bhspin, double inclination_deg, const double* h_angl
 /* mk_shadow_bisect without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (request
  * loops that must not call hipMalloc, harnesses that forbid it).  The same
- * arguments and the same radii, bit for bit, as mk_shadow_bisect; the
+ * arguments and the same radii, bit for bit, as mk_shadow_bisect (in tree
+ * rounds: the band schedule's prediction would allocate host memory); the
  * staging of angles and radii lives in a fixed library-owned array of
  * 3 * MK_NOALLOC_MAX_ANGLES doubles (1.5 MiB) per GPU, held until the process
  * ends.  At most MK_NOALLOC_MAX_ANGLES angles (MK_ERR_ARG beyond).  Calls are

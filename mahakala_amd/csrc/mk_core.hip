@@ -572,9 +572,8 @@ double horizon(double a) { return 1.0 + std::sqrt(1.0 - a * a); }
 using namespace mk;
 
 // ------------------------------------------------ allocation-free staging
-// mk_shadow_bisect_noalloc keeps its staging (cos and sin of the angles in;
-// the predicted radii in and the radii out, one slot) in this fixed array
-// instead of allocating: each GPU holds its
+// mk_shadow_bisect_noalloc keeps its staging (cos and sin of the angles in,
+// radii out) in this fixed array instead of allocating: each GPU holds its
 // own copy (1.5 MiB) until the process ends.  Host code reaches it only
 // through the per-device address from hipGetSymbolAddress; the array's name
 // in host code is not a device address.  The host side of the staging is a
@@ -586,7 +585,7 @@ namespace {
 constexpr int kMaxDevices = 64;
 std::mutex g_noalloc_mu;
 double* g_noalloc_dev[kMaxDevices] = {};
-double g_noalloc_host[3 * MK_NOALLOC_MAX_ANGLES];  // cos, sin, predicted radii
+double g_noalloc_host[2 * MK_NOALLOC_MAX_ANGLES];
 
 // The current device's copy of `scratch`, looked up once per device (called
 // with g_noalloc_mu held).
@@ -1130,12 +1129,10 @@ int mk_shadow_bisect_noalloc(double a, double inc, const double* h_angles, long 
   if (rc) return rc;
   double* hc = g_noalloc_host;
   double* hs = g_noalloc_host + MK_NOALLOC_MAX_ANGLES;
-  double* hr = g_noalloc_host + 2 * MK_NOALLOC_MAX_ANGLES;
   for (long p = 0; p < n; ++p) {
     hc[p] = std::cos(h_angles[p]);
     hs[p] = std::sin(h_angles[p]);
   }
-  predicted_shadow_radii(a, inc, h_angles, n, hr);
   // scratch: cos[0, MAX), sin[MAX, 2 MAX), radii[2 MAX, 3 MAX); every value
   // a launch reads (p < n) is written first, so nothing needs clearing
   double* dc = scr;
@@ -1143,7 +1140,9 @@ int mk_shadow_bisect_noalloc(double a, double inc, const double* h_angles, long 
   double* dr = scr + 2 * MK_NOALLOC_MAX_ANGLES;
   hipError_t e = hipMemcpy(dc, hc, n * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(ds, hs, n * sizeof(double), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_bisect(a, inc, dc, ds, n, max_steps, iters, dist, dr, hr);
+  // no prediction (building it allocates host memory): tree rounds, the
+  // same radii
+  if (e == hipSuccess) e = launch_bisect(a, inc, dc, ds, n, max_steps, iters, dist, dr, nullptr);
   if (e == hipSuccess) e = hipMemcpy(h_radii, dr, n * sizeof(double), hipMemcpyDeviceToHost);
   if (e != hipSuccess) {
     set_error("mk_shadow_bisect_noalloc: %s", hipGetErrorString(e));
