@@ -102,7 +102,7 @@ def make_image(fluid_model, camera_inclination=60, camera_distance=1000, mass_sc
         # device memory at the longest possible history (max_nsteps rows)
         free, _ = _lib.device_memory()
         num_pixels_per_chunk = max(1, min(npx + 10, int(free // 2 // UNFUSED_ROW_BYTES // max_nsteps)))
-    print(num_pixels_per_chunk, npx)
+    print(num_pixels_per_chunk, npx)  # the reference prints the same pair (images.py:76)
     if fluid_gamma is None:
         raise TypeError("fluid_model.fluid_gamma must be set to compute beta (images.py:88)")
 
