@@ -1041,12 +1041,12 @@ static hipError_t launch_bisect(double a, double inc, const double* dc, const do
   // its own: one wave (the band then fits about 0.012 M about the
   // prediction, which is within 0.007 M of the result for |a| <= 0.9) or, for
   // higher spins, where the prediction drifts further (0.02 M at 0.97, more
-  // at high inclination near a = 1), two waves (about 0.03 M).  From 2049
-  // to 4096 angles, blocks of 16 angles of 16 lanes (4096 angles: 10.8
-  // against 12.8 ms in tree rounds; at 2048, 10.9 against 10.0, so the tree
-  // rounds keep 1025-2048).
+  // at high inclination near a = 1), two waves (about 0.03 M).  From 1025
+  // to 4096 angles, 8 angles of 32 lanes per block (2048 angles: 6.9 against
+  // 9.5 ms in tree rounds, 4096 angles 10.1 against 12.8 ms; 16 lanes per
+  // angle: 6.9 and 10.6 ms).
   const double e = bisect_band();
-  if (h_rho && e > 0.0 && iters > 1 && (n <= 1024 || (n > 2048 && n <= 4096))) {
+  if (h_rho && e > 0.0 && iters > 1 && n <= 4096) {
     hipError_t err = hipMemcpy(dr, h_rho, n * sizeof(double), hipMemcpyHostToDevice);
     if (err != hipSuccess) return err;
     const double rEH = horizon(a);
@@ -1055,8 +1055,8 @@ static hipError_t launch_bisect(double a, double inc, const double* dc, const do
     else if (n <= 1024)
       k_bisect_band<128, 1><<<(unsigned)n, 128>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, rEH, e, dr);
     else
-      k_bisect_band<16, 16><<<(unsigned)((n + 15) / 16), 256>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, rEH,
-                                                                 e, dr);
+      k_bisect_band<32, 8><<<(unsigned)((n + 7) / 8), 256>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, rEH,
+                                                               e, dr);
     return hipGetLastError();
   }
   // K halvings per round with 2^K lanes per angle: the fewest rounds with at

@@ -126,11 +126,11 @@ def test_shadow_rounds_of_several_halvings_equal_the_sequential_bisection(monkey
     """Up to 4096 angles, mk_shadow_bisect first integrates, in one round, every
     midpoint of the first halvings whose bracket meets a band about the shadow
     radius predicted from the critical curve (64 lanes per angle up to 1024
-    angles at |a| <= 0.9, 128 above; 16 from 2049 to 4096 angles), replays
+    angles at |a| <= 0.9, 128 above; 32 from 1025 to 4096 angles), replays
     the halvings while the path stays in the band, and runs the rest in tree
-    rounds (2^K - 1 midpoints of the next K halvings at once); from 1025 to
-    2048 and above 4096 angles, tree rounds only (K from the angle count), and
-    one lane per angle from 32768 angles on.  The radii are the same bits either way: with the
+    rounds (2^K - 1 midpoints of the next K halvings at once); above 4096
+    angles, tree rounds only (K from the angle count), and one lane per angle
+    from 32768 angles on.  The radii are the same bits either way: with the
     default band, with a band of 1e-4 M (the path leaves it at most angles, so
     the tree rounds finish them), and with the band off (MK_BISECT_BAND=0,
     tree rounds from the first halving)."""
@@ -162,7 +162,7 @@ def test_shadow_bisection_other_distances_and_depths(dist, err, max_it):
     further off) and at 5, 7 and 24 halvings (the band over every level, and a
     band that fits only a narrow path before the tree rounds): the C
     restatement's radii, bit for bit, for 64 angles (one and two waves per
-    angle) and 2048 angles (tree rounds only)."""
+    angle) and 2048 angles (32 lanes per angle)."""
     from mahakala_amd import _lib
     lib = _lib.lib()
     for n, a, inc in ((64, 0.9, 60.0), (64, 0.95, 80.0), (2048, 0.5, 30.0)):
