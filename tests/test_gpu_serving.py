@@ -138,7 +138,10 @@ def test_persistent_bisection_arriving_during_an_image_only_launch(models):
     with Server("mixed", persistent=True) as srv:
         sid = srv.add_snapshot(models[0])
         first = [srv.submit_image(sid, **kw) for kw in kws]
-        time.sleep(0.002)  # the image-only launch has started (it gathers work for 0.2 ms)
+        t_wait = time.perf_counter() + 30.0
+        while srv.stats()["iterations"] < 1:  # the image-only launch has started
+            assert time.perf_counter() < t_wait, "the first launch did not start"
+            time.sleep(0.0005)
         sh = srv.submit_shadow(0.6, 40.0, ang, max_steps=1500)
         later = [srv.submit_image(sid, **kw) for kw in kws]
         for rid, ref in zip(first + later, refs + refs):
