@@ -1030,10 +1030,10 @@ static void predicted_shadow_radii(double a, double inc_deg, const double* ang, 
 }
 
 // Launches the bisection of n angles (cos and sin in dc, ds; device) into dr
-// on the null stream.  h_rho (n host doubles, may be null): the predicted
-// radii, staged through dr for the band form.
+// on `st`.  h_rho (n host doubles, may be null): the predicted radii, staged
+// through dr for the band form (copied before the call returns).
 static hipError_t launch_bisect(double a, double inc, const double* dc, const double* ds, long n, int max_steps,
-                                int iters, double dist, double* dr, const double* h_rho) {
+                                int iters, double dist, double* dr, const double* h_rho, hipStream_t st) {
   double ci, si;
   cam_angles(inc, &ci, &si);
   // The band form.  The FP64 walk slows with the
@@ -1047,15 +1047,16 @@ static hipError_t launch_bisect(double a, double inc, const double* dc, const do
   // angle: 6.9 and 10.6 ms).
   const double e = bisect_band();
   if (h_rho && e > 0.0 && iters > 1 && n <= 4096) {
-    hipError_t err = hipMemcpy(dr, h_rho, n * sizeof(double), hipMemcpyHostToDevice);
+    hipError_t err = hipMemcpyAsync(dr, h_rho, n * sizeof(double), hipMemcpyHostToDevice, st);
+    if (err == hipSuccess) err = hipStreamSynchronize(st);  // h_rho may be freed after the return
     if (err != hipSuccess) return err;
     const double rEH = horizon(a);
     if (n <= 1024 && std::fabs(a) <= 0.9)
-      k_bisect_band<64, 1><<<(unsigned)n, 64>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, rEH, e, dr);
+      k_bisect_band<64, 1><<<(unsigned)n, 64, 0, st>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, rEH, e, dr);
     else if (n <= 1024)
-      k_bisect_band<128, 1><<<(unsigned)n, 128>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, rEH, e, dr);
+      k_bisect_band<128, 1><<<(unsigned)n, 128, 0, st>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, rEH, e, dr);
     else
-      k_bisect_band<32, 8><<<(unsigned)((n + 7) / 8), 256>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, rEH,
+      k_bisect_band<32, 8><<<(unsigned)((n + 7) / 8), 256, 0, st>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, rEH,
                                                                e, dr);
     return hipGetLastError();
   }
@@ -1068,7 +1069,7 @@ static hipError_t launch_bisect(double a, double inc, const double* dc, const do
   const int rounds = (iters + Kmax - 1) / Kmax;
   const int K = rounds > 0 ? (iters + rounds - 1) / rounds : 1;
   if (K == 1 || iters <= 1) {
-    k_bisect<<<grid_for(n, 64), 64>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, horizon(a), dr);
+    k_bisect<<<grid_for(n, 64), 64, 0, st>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, horizon(a), dr);
   } else {
     // one angle per block (2^K lanes, two or more waves) while the blocks
     // still fit one per CU: fewer waves per CU run the FP64 walk faster
@@ -1080,11 +1081,37 @@ static hipError_t launch_bisect(double a, double inc, const double* dc, const do
       cus = 0;
     const bool one = (1 << K) >= 64 && n <= cus;
     const int block = one ? (1 << K) : ((1 << K) >= 256 ? (1 << K) : 256), G = block >> K;
-    k_bisect_tree<<<(unsigned)((n + G - 1) / G), block>>>(a, ci, si, dist, dc, ds, n, max_steps, iters, horizon(a),
-                                                           K, dr);
+    k_bisect_tree<<<(unsigned)((n + G - 1) / G), block, 0, st>>>(a, ci, si, dist, dc, ds, n, max_steps, iters,
+                                                                  horizon(a), K, dr);
   }
   return hipGetLastError();
 }
+
+}  // extern "C"
+
+namespace mk {
+hipError_t shadow_bisect_on_stream(double a, double inc, const double* h_angles, long n, int max_steps,
+                                   double error_allowed, int max_it, double dist, double* d_stage, double* h_radii,
+                                   hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int iters = bisect_iterations(error_allowed, max_it);
+  std::vector<double> hc(n), hs(n), hr(n);
+  for (long p = 0; p < n; ++p) {
+    hc[p] = std::cos(h_angles[p]);
+    hs[p] = std::sin(h_angles[p]);
+  }
+  predicted_shadow_radii(a, inc, h_angles, n, hr.data());
+  double *dc = d_stage, *ds = d_stage + n, *dr = d_stage + 2 * n;
+  hipError_t e = hipMemcpyAsync(dc, hc.data(), n * sizeof(double), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(ds, hs.data(), n * sizeof(double), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = launch_bisect(a, inc, dc, ds, n, max_steps, iters, dist, dr, hr.data(), st);
+  if (e == hipSuccess) e = hipMemcpyAsync(h_radii, dr, n * sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e;
+}
+}  // namespace mk
+
+extern "C" {
 
 int mk_shadow_bisect(double a, double inc, const double* h_angles, long n, int max_steps, double error_allowed,
                      int max_it, double dist, double* h_radii) {
@@ -1104,7 +1131,7 @@ int mk_shadow_bisect(double a, double inc, const double* h_angles, long n, int m
   if (e == hipSuccess) e = hipMalloc(&dr, n * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(dc, hc.data(), n * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(ds, hs.data(), n * sizeof(double), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_bisect(a, inc, dc, ds, n, max_steps, iters, dist, dr, hr.data());
+  if (e == hipSuccess) e = launch_bisect(a, inc, dc, ds, n, max_steps, iters, dist, dr, hr.data(), nullptr);
   if (e == hipSuccess) e = hipMemcpy(h_radii, dr, n * sizeof(double), hipMemcpyDeviceToHost);
   (void)hipFree(dc);
   (void)hipFree(ds);
@@ -1142,7 +1169,7 @@ int mk_shadow_bisect_noalloc(double a, double inc, const double* h_angles, long 
   if (e == hipSuccess) e = hipMemcpy(ds, hs, n * sizeof(double), hipMemcpyHostToDevice);
   // no prediction (building it allocates host memory): tree rounds, the
   // same radii
-  if (e == hipSuccess) e = launch_bisect(a, inc, dc, ds, n, max_steps, iters, dist, dr, nullptr);
+  if (e == hipSuccess) e = launch_bisect(a, inc, dc, ds, n, max_steps, iters, dist, dr, nullptr, nullptr);
   if (e == hipSuccess) e = hipMemcpy(h_radii, dr, n * sizeof(double), hipMemcpyDeviceToHost);
   if (e != hipSuccess) {
     set_error("mk_shadow_bisect_noalloc: %s", hipGetErrorString(e));

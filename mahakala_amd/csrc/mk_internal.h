@@ -16,6 +16,13 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 
 inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// mk_shadow_bisect on `st` with the caller's device staging d_stage (3 n
+// doubles) and host radii out; returns after the radii are in h_radii.  The
+// same radii, bit for bit.
+hipError_t shadow_bisect_on_stream(double a, double inc, const double* h_angles, long n, int max_steps,
+                                   double error_allowed, int max_it, double dist, double* d_stage, double* h_radii,
+                                   hipStream_t st);
+
 }  // namespace mk
 
 #define MK_HIP(call)                                                                    \

@@ -700,6 +700,7 @@ struct mk_server {
   char* p_out = nullptr;             // output staging (the budgeted arena)
   unsigned long long* d_p = nullptr; // tickets taken, published count seen, busy and total lane-steps
   int* d_pdone = nullptr;
+  double* d_direct = nullptr;        // staging of bisections the idle worker runs itself
   long long pub_tiles = 0;
   long long resume_ticket = 0;       // the next launch's first ticket: the published count at the last stop
   int seg_seq = 0, desc_seq = 0, desc_cap = 0;
@@ -730,6 +731,7 @@ namespace {
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr size_t kMaxUnpolled = size_t(1) << 20;  // completions kept for mk_server_poll
+constexpr int kDirectShadowAngles = 1024;         // bisections an idle server runs itself
 
 bool arena_alloc(mk_server* s, size_t bytes, size_t* off) {
   for (auto it = s->free_blocks.begin(); it != s->free_blocks.end(); ++it) {
@@ -1006,6 +1008,50 @@ bool collect(mk_server* s) {
   return true;
 }
 
+// An idle server (no launch running, nothing in flight) runs a small
+// bisection at the head of its queue itself, in mk_shadow_bisect's band
+// schedule: about one integration chain where the server's launches take one
+// per halving.  The same radii, bit for bit.  (Caller holds mu.)
+bool direct_shadow_next(mk_server* s) {
+  if (s->cfg.flags & MK_SERVER_SHADOW_IN_LAUNCH) return false;
+  if (!s->flight.empty() || s->waiting.empty()) return false;
+  const mk_server::Pending& p = s->waiting.front();
+  return p.kind != kKindImage && p.sh.n_angles <= kDirectShadowAngles;
+}
+
+// Runs the request at the head of the queue, which direct_shadow_next chose,
+// without holding mu; it stays queued until its radii are in its output, so
+// a wait on it does not return early.  Only the worker removes requests from
+// the queue, and a deque keeps references to its elements as others append.
+bool serve_shadow_direct(mk_server* s) {
+  const mk_server::Pending* p;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    p = &s->waiting.front();
+  }
+  const mk_shadow_request& r = p->sh;
+  SRV_HIP_UNLOCKED(hipEventRecord(s->ev0, s->stream));
+  SRV_HIP_UNLOCKED(shadow_bisect_on_stream(r.bhspin, r.inclination, p->angles.data(), r.n_angles, r.max_steps,
+                                           r.error_allowed, r.max_it, r.distance, s->d_direct, p->h_out,
+                                           s->stream));
+  SRV_HIP_UNLOCKED(hipEventRecord(s->ev1, s->stream));
+  SRV_HIP_UNLOCKED(hipEventSynchronize(s->ev1));
+  float gpu_ms = 0;
+  SRV_HIP_UNLOCKED(hipEventElapsedTime(&gpu_ms, s->ev0, s->ev1));
+  std::lock_guard<std::mutex> lk(s->mu);
+  const double ms = std::chrono::duration<double, std::milli>(Clock::now() - p->t0).count();
+  if (s->unpolled.size() >= kMaxUnpolled)
+    s->unpolled.erase(s->unpolled.begin(), s->unpolled.begin() + kMaxUnpolled / 2);
+  s->unpolled.emplace_back(p->id, ms);
+  s->st.gpu_ms += gpu_ms;
+  ++s->st.admitted;
+  ++s->st.completed;
+  s->st.items_done += r.n_angles;
+  s->waiting.pop_front();
+  s->cv_done.notify_all();
+  return true;
+}
+
 void worker_main(mk_server* s) {
   if (hipSetDevice(s->dev) != hipSuccess) {
     std::lock_guard<std::mutex> lk(s->mu);
@@ -1017,6 +1063,11 @@ void worker_main(mk_server* s) {
   while (!s->stop.load()) {
     {
       std::unique_lock<std::mutex> lk(s->mu);
+      if (direct_shadow_next(s)) {
+        lk.unlock();
+        if (!serve_shadow_direct(s)) break;
+        continue;
+      }
       if (!admit(s)) break;
       if (s->flight.empty()) {
         s->cv_work.wait_for(lk, std::chrono::milliseconds(20));
@@ -1256,6 +1307,12 @@ void worker_persistent(mk_server* s) {
       break;
     }
     collect_persistent(s);
+    if (!s->launched && direct_shadow_next(s)) {
+      lk.unlock();
+      ok = serve_shadow_direct(s);
+      last_busy = Clock::now();
+      continue;
+    }
     if (s->draining && s->flight.empty()) {
       if (s->launched) {
         stop_persistent(s);  // the launch ends within about a tenth of a millisecond
@@ -1334,6 +1391,7 @@ int mk_server_create(const mk_server_config* cfg, mk_server** out) {
   s->arena_bytes = cfg->hbm_budget > 0 ? cfg->hbm_budget : ((size_t)(s->persistent ? 1 : 4) << 30);
   const long L = s->L;
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&s->d_direct, 3 * kDirectShadowAngles * sizeof(double));
   if (s->persistent) {
     // seg_cap > Rmax: a segment's entry is overwritten only long after its
     // request finished (at most Rmax requests are in flight)
@@ -1551,6 +1609,7 @@ int mk_server_destroy(mk_server* s) {
   (void)hipFree(s->d_arena);
   (void)hipFree(s->d_p);
   (void)hipFree(s->d_pdone);
+  (void)hipFree(s->d_direct);
   for (void* h : {(void*)s->p_ctl, (void*)s->p_segs, (void*)s->p_desc, (void*)s->p_fin, (void*)s->p_in,
                   (void*)s->p_out})
     if (h) (void)hipHostFree(h);

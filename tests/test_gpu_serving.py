@@ -90,12 +90,17 @@ def test_served_images_equal_render_and_do_not_depend_on_the_mix(models, precisi
             assert abs(alone[k].sum() - ref64.sum()) <= 1e-3 * ref64.sum()
 
 
+@pytest.mark.parametrize("in_launch", [True, False])
 @pytest.mark.parametrize("persistent", [False, True])
-def test_served_shadow_equals_bisection_kernel(persistent):
+def test_served_shadow_equals_bisection_kernel(persistent, in_launch):
+    """Bisections served inside the launches, one lane per angle
+    (in_launch), and run by the idle server itself in the band schedule:
+    mk_shadow_bisect's radii either way."""
     ang = np.linspace(0, 2 * np.pi, 40, endpoint=False)
     ref_a = find_shadow_bisection_angles(0.9, 60.0, ang, max_steps=2000)
     ref_b = find_shadow_bisection_angles(-0.4, 20.0, ang[::3], max_steps=1500, error_allowed=1e-2)
-    with Server("mixed", lanes=1024, steps_per_launch=128, persistent=persistent) as srv:
+    with Server("mixed", lanes=1024, steps_per_launch=128, persistent=persistent,
+                shadow_in_launch=in_launch) as srv:
         a = srv.submit_shadow(0.9, 60.0, ang, max_steps=2000)
         b = srv.submit_shadow(-0.4, 20.0, ang[::3], max_steps=1500, error_allowed=1e-2)
         c = srv.submit_shadow(0.1, 45.0, ang[:5], error_allowed=20.0)  # no halving: 0.5
@@ -108,13 +113,44 @@ def test_served_shadow_equals_bisection_kernel(persistent):
     assert np.all((ref_a > 3) & (ref_a < 7))
 
 
+@pytest.mark.parametrize("persistent", [False, True])
+def test_idle_server_runs_small_bisections_itself(models, persistent):
+    """An idle server runs a bisection of up to 1024 angles itself, in the
+    band schedule, without a launch; a bisection queued behind images, and
+    one of more than 1024 angles, go through the launches.  Every radius is
+    mk_shadow_bisect's, and a wait on a request returns only once it is done."""
+    small = np.linspace(0, 2 * np.pi, 64, endpoint=False)
+    large = np.linspace(0, 2 * np.pi, 1100, endpoint=False)
+    ref_s = find_shadow_bisection_angles(0.9, 60.0, small)
+    ref_l = find_shadow_bisection_angles(0.5, 30.0, large, max_steps=1000)
+    kw = _img_kw(128, 0.5, 30.0, 230e9)
+    ref_i = _render(models[0], kw["resolution"], kw["bhspin"], kw["inclination"], kw["observing_frequency"], "mixed")
+    with Server("mixed", persistent=persistent) as srv:
+        sid = srv.add_snapshot(models[0])
+        t0 = time.perf_counter()
+        np.testing.assert_array_equal(srv.result(srv.submit_shadow(0.9, 60.0, small), timeout=60), ref_s)
+        t_small = time.perf_counter() - t0
+        st = srv.stats()
+        assert st["iterations"] == 0 and st["completed"] == 1  # no launch
+        img = srv.submit_image(sid, **kw)
+        sh = srv.submit_shadow(0.9, 60.0, small)     # behind an image: through a launch
+        big = srv.submit_shadow(0.5, 30.0, large, max_steps=1000)  # over 1024 angles: through a launch
+        np.testing.assert_array_equal(srv.result(img, timeout=60), ref_i)
+        np.testing.assert_array_equal(srv.result(sh, timeout=60), ref_s)
+        np.testing.assert_array_equal(srv.result(big, timeout=60), ref_l)
+        time.sleep(0.1)
+        st = srv.stats()
+    print(f"[serving] idle 64-angle bisection: {t_small * 1e3:.2f} ms; {st['iterations']} launches after")
+    assert st["completed"] == 4 and st["in_flight"] == 0 and st["iterations"] >= 1
+
+
 def test_persistent_launch_ends_when_idle_and_resumes_its_tickets():
     """The persistent launch ends after the server has been idle; its waves
     leave holding unpublished tickets, and the next launch resumes at the
     published count, so requests after each relaunch are served in full."""
     ang = np.linspace(0, 2 * np.pi, 200, endpoint=False)  # 4 tiles, the last one partial
     ref = find_shadow_bisection_angles(0.7, 50.0, ang, max_steps=1500)
-    with Server("mixed", lanes=4096, persistent=True) as srv:
+    with Server("mixed", lanes=4096, persistent=True, shadow_in_launch=True) as srv:
         for k in range(4):
             print(f"[serving] persistent relaunch round {k}", flush=True)
             rid = srv.submit_shadow(0.7, 50.0, ang, max_steps=1500)
