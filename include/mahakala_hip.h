@@ -282,6 +282,11 @@ typedef struct mk_server_config {
  * instead of one per halving (64 angles: 3.5 against 35 ms); the same radii,
  * bit for bit.  Requests submitted meanwhile wait until it ends. */
 #define MK_SERVER_SHADOW_IN_LAUNCH 2
+/* mk_server_config.flags: images always run inside the server's launches.
+ * Without it, an idle server whose queue holds one image of up to 256^2 and
+ * nothing else renders it itself through mk_render (three waves per tile).
+ * The same image, bit for bit. */
+#define MK_SERVER_IMAGE_IN_LAUNCH 4
 
 typedef struct mk_image_request {
   int32_t snapshot;         /* id from mk_server_add_snapshot */

@@ -701,6 +701,7 @@ struct mk_server {
   unsigned long long* d_p = nullptr; // tickets taken, published count seen, busy and total lane-steps
   int* d_pdone = nullptr;
   double* d_direct = nullptr;        // staging of bisections the idle worker runs itself
+  double* d_direct_img = nullptr;    // and of its images
   long long pub_tiles = 0;
   long long resume_ticket = 0;       // the next launch's first ticket: the published count at the last stop
   int seg_seq = 0, desc_seq = 0, desc_cap = 0;
@@ -732,6 +733,7 @@ size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr size_t kMaxUnpolled = size_t(1) << 20;  // completions kept for mk_server_poll
 constexpr int kDirectShadowAngles = 1024;         // bisections an idle server runs itself
+constexpr int kDirectImageRes = 256;              // and images, when one is all it has
 
 bool arena_alloc(mk_server* s, size_t bytes, size_t* off) {
   for (auto it = s->free_blocks.begin(); it != s->free_blocks.end(); ++it) {
@@ -813,27 +815,34 @@ std::vector<int> tile_order(mk_server* s, int res, double fov, double lo, double
 
 // Descriptor fields of a request, all but its memory (out, order, cang,
 // sang); cs receives a bisection's cos and sin table (cos then sin).
+// mk_render's parameters of an image request (make_image's fixed div, tol,
+// pitch and sigma cut).
+mk_render_params image_params(const mk_server* s, const mk_image_request& r) {
+  mk_render_params rpar{};
+  rpar.bhspin = r.bhspin;
+  rpar.inclination = r.inclination;
+  rpar.distance = r.distance;
+  rpar.fov = r.fov;
+  rpar.mass_scale = r.mass_scale;
+  rpar.M_bh = r.M_bh;
+  rpar.r_high = r.r_high;
+  rpar.fluid_gamma = r.fluid_gamma;
+  rpar.observing_frequency = r.observing_frequency;
+  rpar.div = 40.0;
+  rpar.tol = 1e-4;
+  rpar.fallback_pitch = M_PI / 3.0;
+  rpar.sigma_cut = 100.0;
+  rpar.resolution = r.resolution;
+  rpar.max_nsteps = r.max_nsteps;
+  rpar.precision = s->cfg.precision;
+  return rpar;
+}
+
 void fill_desc(mk_server* s, const mk_server::Pending& p, ReqDev& q, std::vector<double>* cs) {
   std::memset(&q, 0, sizeof q);
   if (p.kind == kKindImage) {
-    mk_render_params rpar{};
     const mk_image_request& r = p.img;
-    rpar.bhspin = r.bhspin;
-    rpar.inclination = r.inclination;
-    rpar.distance = r.distance;
-    rpar.fov = r.fov;
-    rpar.mass_scale = r.mass_scale;
-    rpar.M_bh = r.M_bh;
-    rpar.r_high = r.r_high;
-    rpar.fluid_gamma = r.fluid_gamma;
-    rpar.observing_frequency = r.observing_frequency;
-    rpar.div = 40.0;
-    rpar.tol = 1e-4;
-    rpar.fallback_pitch = M_PI / 3.0;
-    rpar.sigma_cut = 100.0;
-    rpar.resolution = r.resolution;
-    rpar.max_nsteps = r.max_nsteps;
-    rpar.precision = s->cfg.precision;
+    const mk_render_params rpar = image_params(s, r);
     RenderDev rd;
     fill_render_dev(&rpar, &rd);
     q.kind = kKindImage;
@@ -1008,32 +1017,55 @@ bool collect(mk_server* s) {
   return true;
 }
 
-// An idle server (no launch running, nothing in flight) runs a small
-// bisection at the head of its queue itself, in mk_shadow_bisect's band
-// schedule: about one integration chain where the server's launches take one
-// per halving.  The same radii, bit for bit.  (Caller holds mu.)
-bool direct_shadow_next(mk_server* s) {
-  if (s->cfg.flags & MK_SERVER_SHADOW_IN_LAUNCH) return false;
+// An idle server (no launch running, nothing in flight) runs a small request
+// at the head of its queue itself: a bisection of up to 1024 angles in
+// mk_shadow_bisect's band schedule (about one integration chain where the
+// server's launches take one per halving), and, when it is the only request
+// queued, an image of up to 256^2 through mk_render (three waves per 8x8
+// tile; row-major tiles, which at one tile per SIMD cost nothing).  The same
+// results, bit for bit.  (Caller holds mu.)
+bool direct_next(mk_server* s) {
   if (!s->flight.empty() || s->waiting.empty()) return false;
   const mk_server::Pending& p = s->waiting.front();
-  return p.kind != kKindImage && p.sh.n_angles <= kDirectShadowAngles;
+  if (p.kind != kKindImage)
+    return !(s->cfg.flags & MK_SERVER_SHADOW_IN_LAUNCH) && p.sh.n_angles <= kDirectShadowAngles;
+  return !(s->cfg.flags & MK_SERVER_IMAGE_IN_LAUNCH) && s->waiting.size() == 1 &&
+         p.img.resolution <= kDirectImageRes;
 }
 
-// Runs the request at the head of the queue, which direct_shadow_next chose,
-// without holding mu; it stays queued until its radii are in its output, so
+// Runs the request at the head of the queue, which direct_next chose,
+// without holding mu; it stays queued until its result is in its output, so
 // a wait on it does not return early.  Only the worker removes requests from
 // the queue, and a deque keeps references to its elements as others append.
-bool serve_shadow_direct(mk_server* s) {
+bool serve_direct(mk_server* s) {
   const mk_server::Pending* p;
+  const mk_snapshot* snap = nullptr;
   {
     std::lock_guard<std::mutex> lk(s->mu);
     p = &s->waiting.front();
+    if (p->kind == kKindImage) snap = s->snaps[p->img.snapshot];
   }
-  const mk_shadow_request& r = p->sh;
+  long items = 0;
   SRV_HIP_UNLOCKED(hipEventRecord(s->ev0, s->stream));
-  SRV_HIP_UNLOCKED(shadow_bisect_on_stream(r.bhspin, r.inclination, p->angles.data(), r.n_angles, r.max_steps,
-                                           r.error_allowed, r.max_it, r.distance, s->d_direct, p->h_out,
-                                           s->stream));
+  if (p->kind == kKindImage) {
+    mk_render_params rp = image_params(s, p->img);
+    rp.flags |= MK_RENDER_ROW_MAJOR;
+    items = (long)rp.resolution * rp.resolution;
+    if (mk_render(snap, &rp, s->d_direct_img, nullptr, s->stream) != MK_OK) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->status = MK_ERR_HIP;
+      s->err = std::string("direct image: ") + mk_last_error();
+      return false;
+    }
+    SRV_HIP_UNLOCKED(hipMemcpyAsync(p->h_out, s->d_direct_img, items * sizeof(double), hipMemcpyDeviceToHost,
+                                    s->stream));
+  } else {
+    const mk_shadow_request& r = p->sh;
+    items = r.n_angles;
+    SRV_HIP_UNLOCKED(shadow_bisect_on_stream(r.bhspin, r.inclination, p->angles.data(), r.n_angles, r.max_steps,
+                                             r.error_allowed, r.max_it, r.distance, s->d_direct, p->h_out,
+                                             s->stream));
+  }
   SRV_HIP_UNLOCKED(hipEventRecord(s->ev1, s->stream));
   SRV_HIP_UNLOCKED(hipEventSynchronize(s->ev1));
   float gpu_ms = 0;
@@ -1046,7 +1078,7 @@ bool serve_shadow_direct(mk_server* s) {
   s->st.gpu_ms += gpu_ms;
   ++s->st.admitted;
   ++s->st.completed;
-  s->st.items_done += r.n_angles;
+  s->st.items_done += items;
   s->waiting.pop_front();
   s->cv_done.notify_all();
   return true;
@@ -1063,9 +1095,9 @@ void worker_main(mk_server* s) {
   while (!s->stop.load()) {
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      if (direct_shadow_next(s)) {
+      if (direct_next(s)) {
         lk.unlock();
-        if (!serve_shadow_direct(s)) break;
+        if (!serve_direct(s)) break;
         continue;
       }
       if (!admit(s)) break;
@@ -1307,9 +1339,9 @@ void worker_persistent(mk_server* s) {
       break;
     }
     collect_persistent(s);
-    if (!s->launched && direct_shadow_next(s)) {
+    if (!s->launched && direct_next(s)) {
       lk.unlock();
-      ok = serve_shadow_direct(s);
+      ok = serve_direct(s);
       last_busy = Clock::now();
       continue;
     }
@@ -1392,6 +1424,7 @@ int mk_server_create(const mk_server_config* cfg, mk_server** out) {
   const long L = s->L;
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&s->d_direct, 3 * kDirectShadowAngles * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&s->d_direct_img, (size_t)kDirectImageRes * kDirectImageRes * sizeof(double));
   if (s->persistent) {
     // seg_cap > Rmax: a segment's entry is overwritten only long after its
     // request finished (at most Rmax requests are in flight)
@@ -1610,6 +1643,7 @@ int mk_server_destroy(mk_server* s) {
   (void)hipFree(s->d_p);
   (void)hipFree(s->d_pdone);
   (void)hipFree(s->d_direct);
+  (void)hipFree(s->d_direct_img);
   for (void* h : {(void*)s->p_ctl, (void*)s->p_segs, (void*)s->p_desc, (void*)s->p_fin, (void*)s->p_in,
                   (void*)s->p_out})
     if (h) (void)hipHostFree(h);

@@ -29,7 +29,7 @@ class Server:
     """Owner of an ``mk_server`` (destroyed by ``close`` or on collection)."""
 
     def __init__(self, precision="plain64", lanes=0, steps_per_launch=0, max_requests=0, hbm_budget=0,
-                 use_graph=True, persistent=True, shadow_in_launch=False):
+                 use_graph=True, persistent=True, shadow_in_launch=False, image_in_launch=False):
         """persistent: one resident launch whose waves take newly admitted
         work without a launch boundary (MK_SERVER_PERSISTENT; lower latency
         and higher throughput on every stream measured); False: a launch per
@@ -38,10 +38,13 @@ class Server:
         (MK_SERVER_SHADOW_IN_LAUNCH); by default an idle server runs a
         bisection of up to 1024 angles itself, in the band schedule of
         find_shadow_bisection_angles (same radii, about a tenth of the
-        latency)."""
+        latency).  image_in_launch: images always run inside the launches
+        (MK_SERVER_IMAGE_IN_LAUNCH); by default an idle server whose queue
+        holds one image of up to 256^2 and nothing else renders it itself
+        through mk_render (the same image)."""
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
-        flags = (1 if persistent else 0) | (2 if shadow_in_launch else 0)
+        flags = (1 if persistent else 0) | (2 if shadow_in_launch else 0) | (4 if image_in_launch else 0)
         cfg = _lib.mk_server_config(PRECISIONS[precision], int(lanes), int(steps_per_launch), int(max_requests),
                                     int(hbm_budget), int(bool(use_graph)), flags)
         h = _lib.vp()

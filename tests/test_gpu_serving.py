@@ -57,7 +57,7 @@ def test_served_images_equal_render_and_do_not_depend_on_the_mix(models, precisi
     # few lanes and short launches: rays of many requests share waves and
     # each slot is reused many times (the persistent form: one launch whose
     # waves take tiles of every request as they are published)
-    kw = dict(lanes=2048, steps_per_launch=64, persistent=persistent)
+    kw = dict(lanes=2048, steps_per_launch=64, persistent=persistent, image_in_launch=True)
     with Server(precision, **kw) as srv:
         sids = [srv.add_snapshot(m) for m in models]
         for k, (res, spin, inc, nu, s) in enumerate(MIX):
@@ -125,7 +125,7 @@ def test_idle_server_runs_small_bisections_itself(models, persistent):
     ref_l = find_shadow_bisection_angles(0.5, 30.0, large, max_steps=1000)
     kw = _img_kw(128, 0.5, 30.0, 230e9)
     ref_i = _render(models[0], kw["resolution"], kw["bhspin"], kw["inclination"], kw["observing_frequency"], "mixed")
-    with Server("mixed", persistent=persistent) as srv:
+    with Server("mixed", persistent=persistent, image_in_launch=True) as srv:
         sid = srv.add_snapshot(models[0])
         t0 = time.perf_counter()
         np.testing.assert_array_equal(srv.result(srv.submit_shadow(0.9, 60.0, small), timeout=60), ref_s)
@@ -142,6 +142,32 @@ def test_idle_server_runs_small_bisections_itself(models, persistent):
         st = srv.stats()
     print(f"[serving] idle 64-angle bisection: {t_small * 1e3:.2f} ms; {st['iterations']} launches after")
     assert st["completed"] == 4 and st["in_flight"] == 0 and st["iterations"] >= 1
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_idle_server_renders_a_lone_small_image_itself(models, persistent):
+    """An idle server whose queue holds one image of up to 256^2 renders it
+    itself through mk_render, without a launch; two images queued together,
+    and a larger image, go through the launches.  Every image is mk_render's,
+    bit for bit, in both precisions' default path (mixed here)."""
+    kws = [_img_kw(256, 0.5, 30.0, 230e9), _img_kw(128, -0.6, 70.0, 86e9), _img_kw(320, 0.9, 60.0, 345e9)]
+    refs = [_render(models[0], kw["resolution"], kw["bhspin"], kw["inclination"], kw["observing_frequency"], "mixed")
+            for kw in kws]
+    with Server("mixed", persistent=persistent) as srv:
+        sid = srv.add_snapshot(models[0])
+        for kw, ref in zip(kws[:2], refs[:2]):
+            t0 = time.perf_counter()
+            np.testing.assert_array_equal(srv.result(srv.submit_image(sid, **kw), timeout=60), ref)
+            print(f"[serving] lone {kw['resolution']}^2 image: {(time.perf_counter() - t0) * 1e3:.2f} ms")
+        st = srv.stats()
+        assert st["iterations"] == 0 and st["completed"] == 2
+        pair = [srv.submit_image(sid, **kw) for kw in kws[:2]]   # queued together: a launch
+        big = srv.submit_image(sid, **kws[2])                     # over 256^2: a launch
+        for rid, ref in zip(pair + [big], refs[:2] + refs[2:]):
+            np.testing.assert_array_equal(srv.result(rid, timeout=60), ref)
+        time.sleep(0.1)
+        st = srv.stats()
+    assert st["completed"] == 5 and st["in_flight"] == 0 and st["iterations"] >= 1
 
 
 def test_persistent_launch_ends_when_idle_and_resumes_its_tickets():
@@ -171,7 +197,7 @@ def test_persistent_bisection_arriving_during_an_image_only_launch(models):
     kws = [_img_kw(256, 0.5, 30.0, 230e9), _img_kw(192, -0.3, 60.0, 86e9), _img_kw(128, 0.8, 20.0, 345e9)]
     refs = [_render(models[0], kw["resolution"], kw["bhspin"], kw["inclination"], kw["observing_frequency"], "mixed")
             for kw in kws]
-    with Server("mixed", persistent=True) as srv:
+    with Server("mixed", persistent=True, shadow_in_launch=True, image_in_launch=True) as srv:
         sid = srv.add_snapshot(models[0])
         first = [srv.submit_image(sid, **kw) for kw in kws]
         t_wait = time.perf_counter() + 30.0
@@ -233,7 +259,7 @@ def test_budget_back_pressure_and_errors(models, persistent):
     # (36 tiles x 4 bytes, aligned to 256; the persistent form keeps the
     # order outside the budget)
     need = res * res * 8 + 256
-    with Server("mixed", hbm_budget=need, lanes=4096, persistent=persistent) as srv:
+    with Server("mixed", hbm_budget=need, lanes=4096, persistent=persistent, image_in_launch=True) as srv:
         sid = srv.add_snapshot(models[0])
         ref = _render(models[0], res, 0.5, 30.0, 230e9, "mixed")
         rids = [srv.submit_image(sid, **_img_kw(res, 0.5, 30.0, 230e9)) for _ in range(3)]
