@@ -684,6 +684,30 @@ def main():
             out["serving_persistent" if pers and len(forms) > 1 else "serving"] = sv
         sv = out["serving"]
         out["latency_ms"] = dict(sv["latency_ms"], what="submit-to-result latency of the serving stream")
+        # one request at a time through an idle server (default flags, and with
+        # every request kept inside the launches): submit-to-result medians
+        from mahakala_amd.serving import Server
+        ang64 = np.arange(64) / 64 * 2 * np.pi
+        idle = {}
+        for name, kw in (("default", {}), ("in_launch", {"image_in_launch": True, "shadow_in_launch": True})):
+            with Server(head if head != "mixed-zone" else "mixed", **kw) as srv:
+                sid = srv.add_snapshot(model)
+                t_img, t_sh = [], []
+                for _ in range(6):
+                    time.sleep(0.05)  # longer than the idle limit: no launch running
+                    t1 = time.perf_counter()
+                    srv.result(srv.submit_image(sid, a, inc, resolution=256, observing_frequency=nu), timeout=60)
+                    t_img.append(time.perf_counter() - t1)
+                    time.sleep(0.05)
+                    t1 = time.perf_counter()
+                    srv.result(srv.submit_shadow(0.9, 60.0, ang64), timeout=60)
+                    t_sh.append(time.perf_counter() - t1)
+            idle[name] = {"image_256_ms": round(float(np.median(t_img[1:])) * 1e3, 3),
+                          "shadow_64_angles_ms": round(float(np.median(t_sh[1:])) * 1e3, 3)}
+        out["serving_idle"] = dict(idle, what="one request at a time through an idle server (persistent form): a "
+                                              "256^2 image of the bench scene and a 64-angle bisection (a=0.9, "
+                                              "i=60 deg); in_launch keeps both inside the server's launches")
+        print(f"[bench] idle server: {idle}", file=sys.stderr, flush=True)
         del model1, ds1
 
     if not args.no_heavy:
