@@ -283,9 +283,10 @@ typedef struct mk_server_config {
  * bit for bit.  Requests submitted meanwhile wait until it ends. */
 #define MK_SERVER_SHADOW_IN_LAUNCH 2
 /* mk_server_config.flags: images always run inside the server's launches.
- * Without it, an idle server whose queue holds one image of up to 256^2 and
- * nothing else renders it itself through mk_render (three waves per tile).
- * The same image, bit for bit. */
+ * Without it, an idle server whose queue holds one image of up to 512^2 and
+ * nothing else renders it itself through mk_render (up to 256^2 three waves
+ * per tile).  The same image, bit for bit.  Above 256^2 that call builds
+ * mk_render's tile order for the view, as mk_render's own first call does. */
 #define MK_SERVER_IMAGE_IN_LAUNCH 4
 
 typedef struct mk_image_request {

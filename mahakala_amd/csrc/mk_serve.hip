@@ -733,7 +733,7 @@ size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr size_t kMaxUnpolled = size_t(1) << 20;  // completions kept for mk_server_poll
 constexpr int kDirectShadowAngles = 1024;         // bisections an idle server runs itself
-constexpr int kDirectImageRes = 256;              // and images, when one is all it has
+constexpr int kDirectImageRes = 512;              // and images, when one is all it has
 
 bool arena_alloc(mk_server* s, size_t bytes, size_t* off) {
   for (auto it = s->free_blocks.begin(); it != s->free_blocks.end(); ++it) {
@@ -1021,9 +1021,10 @@ bool collect(mk_server* s) {
 // at the head of its queue itself: a bisection of up to 1024 angles in
 // mk_shadow_bisect's band schedule (about one integration chain where the
 // server's launches take one per halving), and, when it is the only request
-// queued, an image of up to 256^2 through mk_render (three waves per 8x8
-// tile; row-major tiles, which at one tile per SIMD cost nothing).  The same
-// results, bit for bit.  (Caller holds mu.)
+// queued, an image of up to 512^2 through mk_render (three waves per 8x8 tile
+// up to 256^2, with row-major tiles, which at one tile per SIMD cost nothing;
+// above, mk_render's ring-first tile order, built once per view and kept).
+// The same results, bit for bit.  (Caller holds mu.)
 bool direct_next(mk_server* s) {
   if (!s->flight.empty() || s->waiting.empty()) return false;
   const mk_server::Pending& p = s->waiting.front();
@@ -1049,7 +1050,7 @@ bool serve_direct(mk_server* s) {
   SRV_HIP_UNLOCKED(hipEventRecord(s->ev0, s->stream));
   if (p->kind == kKindImage) {
     mk_render_params rp = image_params(s, p->img);
-    rp.flags |= MK_RENDER_ROW_MAJOR;
+    if (rp.resolution <= 256) rp.flags |= MK_RENDER_ROW_MAJOR;  // one tile per SIMD: order costs nothing
     items = (long)rp.resolution * rp.resolution;
     if (mk_render(snap, &rp, s->d_direct_img, nullptr, s->stream) != MK_OK) {
       std::lock_guard<std::mutex> lk(s->mu);

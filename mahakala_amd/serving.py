@@ -40,7 +40,7 @@ class Server:
         find_shadow_bisection_angles (same radii, about a tenth of the
         latency).  image_in_launch: images always run inside the launches
         (MK_SERVER_IMAGE_IN_LAUNCH); by default an idle server whose queue
-        holds one image of up to 256^2 and nothing else renders it itself
+        holds one image of up to 512^2 and nothing else renders it itself
         through mk_render (the same image)."""
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")

@@ -146,28 +146,29 @@ def test_idle_server_runs_small_bisections_itself(models, persistent):
 
 @pytest.mark.parametrize("persistent", [False, True])
 def test_idle_server_renders_a_lone_small_image_itself(models, persistent):
-    """An idle server whose queue holds one image of up to 256^2 renders it
+    """An idle server whose queue holds one image of up to 512^2 renders it
     itself through mk_render, without a launch; two images queued together,
     and a larger image, go through the launches.  Every image is mk_render's,
-    bit for bit, in both precisions' default path (mixed here)."""
-    kws = [_img_kw(256, 0.5, 30.0, 230e9), _img_kw(128, -0.6, 70.0, 86e9), _img_kw(320, 0.9, 60.0, 345e9)]
+    bit for bit."""
+    kws = [_img_kw(256, 0.5, 30.0, 230e9), _img_kw(128, -0.6, 70.0, 86e9), _img_kw(512, 0.9, 60.0, 345e9),
+           _img_kw(640, 0.3, 45.0, 230e9)]
     refs = [_render(models[0], kw["resolution"], kw["bhspin"], kw["inclination"], kw["observing_frequency"], "mixed")
             for kw in kws]
     with Server("mixed", persistent=persistent) as srv:
         sid = srv.add_snapshot(models[0])
-        for kw, ref in zip(kws[:2], refs[:2]):
+        for kw, ref in zip(kws[:3], refs[:3]):
             t0 = time.perf_counter()
             np.testing.assert_array_equal(srv.result(srv.submit_image(sid, **kw), timeout=60), ref)
             print(f"[serving] lone {kw['resolution']}^2 image: {(time.perf_counter() - t0) * 1e3:.2f} ms")
         st = srv.stats()
-        assert st["iterations"] == 0 and st["completed"] == 2
+        assert st["iterations"] == 0 and st["completed"] == 3
         pair = [srv.submit_image(sid, **kw) for kw in kws[:2]]   # queued together: a launch
-        big = srv.submit_image(sid, **kws[2])                     # over 256^2: a launch
-        for rid, ref in zip(pair + [big], refs[:2] + refs[2:]):
+        big = srv.submit_image(sid, **kws[3])                     # over 512^2: a launch
+        for rid, ref in zip(pair + [big], refs[:2] + refs[3:]):
             np.testing.assert_array_equal(srv.result(rid, timeout=60), ref)
         time.sleep(0.1)
         st = srv.stats()
-    assert st["completed"] == 5 and st["in_flight"] == 0 and st["iterations"] >= 1
+    assert st["completed"] == 6 and st["in_flight"] == 0 and st["iterations"] >= 1
 
 
 def test_persistent_launch_ends_when_idle_and_resumes_its_tickets():
