@@ -813,8 +813,6 @@ std::vector<int> tile_order(mk_server* s, int res, double fov, double lo, double
   return ord;
 }
 
-// Descriptor fields of a request, all but its memory (out, order, cang,
-// sang); cs receives a bisection's cos and sin table (cos then sin).
 // mk_render's parameters of an image request (make_image's fixed div, tol,
 // pitch and sigma cut).
 mk_render_params image_params(const mk_server* s, const mk_image_request& r) {
@@ -838,6 +836,8 @@ mk_render_params image_params(const mk_server* s, const mk_image_request& r) {
   return rpar;
 }
 
+// Descriptor fields of a request, all but its memory (out, order, cang,
+// sang); cs receives a bisection's cos and sin table (cos then sin).
 void fill_desc(mk_server* s, const mk_server::Pending& p, ReqDev& q, std::vector<double>* cs) {
   std::memset(&q, 0, sizeof q);
   if (p.kind == kKindImage) {
