@@ -293,13 +293,13 @@ typedef struct mk_image_request {
   int32_t snapshot;         /* id from mk_server_add_snapshot */
   int32_t resolution;
   int32_t max_nsteps;       /* 10000 in make_image */
-  int32_t reserved;
+  int32_t priority;         /* higher is admitted first; equal: submit order (0 by default) */
   double bhspin, inclination, distance, fov, observing_frequency, mass_scale, M_bh, r_high, fluid_gamma;
 } mk_image_request;
 
 typedef struct mk_shadow_request {
   double bhspin, inclination, distance, error_allowed;
-  int32_t n_angles, max_steps, max_it, reserved;
+  int32_t n_angles, max_steps, max_it, priority; /* priority as for images */
   const double* h_angles;   /* copied at submit */
 } mk_shadow_request;
 
@@ -329,6 +329,11 @@ int mk_server_poll(mk_server* srv, int64_t* h_ids, double* h_latency_ms, int32_t
 /* block until request id (or all, id < 0) completes; MK_ERR_TIMEOUT after
  * timeout_s seconds, MK_ERR_ARG for an id never issued */
 int mk_server_wait(mk_server* srv, int64_t id, double timeout_s);
+/* Removes a request that is still queued (not yet admitted to a launch nor
+ * being run): *h_cancelled = 1, its output is never written, and a wait on it
+ * returns MK_ERR_STATE; otherwise *h_cancelled = 0 (it is running or done).
+ * No counterpart in the reference. */
+int mk_server_cancel(mk_server* srv, int64_t id, int32_t* h_cancelled);
 int mk_server_get_stats(mk_server* srv, mk_server_stats* h_stats);
 int mk_server_destroy(mk_server* srv);
 

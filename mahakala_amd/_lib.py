@@ -46,7 +46,7 @@ class mk_server_config(C.Structure):
 
 class mk_image_request(C.Structure):
     _fields_ = [
-        ("snapshot", i32), ("resolution", i32), ("max_nsteps", i32), ("reserved", i32),
+        ("snapshot", i32), ("resolution", i32), ("max_nsteps", i32), ("priority", i32),
         ("bhspin", dbl), ("inclination", dbl), ("distance", dbl), ("fov", dbl),
         ("observing_frequency", dbl), ("mass_scale", dbl), ("M_bh", dbl), ("r_high", dbl),
         ("fluid_gamma", dbl),
@@ -56,7 +56,7 @@ class mk_image_request(C.Structure):
 class mk_shadow_request(C.Structure):
     _fields_ = [
         ("bhspin", dbl), ("inclination", dbl), ("distance", dbl), ("error_allowed", dbl),
-        ("n_angles", i32), ("max_steps", i32), ("max_it", i32), ("reserved", i32),
+        ("n_angles", i32), ("max_steps", i32), ("max_it", i32), ("priority", i32),
         ("h_angles", vp),
     ]
 
@@ -125,6 +125,7 @@ _SIGS = {
     "mk_server_submit_shadow": [vp, C.POINTER(mk_shadow_request), vp, C.POINTER(C.c_int64)],
     "mk_server_poll": [vp, vp, vp, i32, C.POINTER(i32)],
     "mk_server_wait": [vp, C.c_int64, dbl],
+    "mk_server_cancel": [vp, C.c_int64, C.POINTER(i32)],
     "mk_server_get_stats": [vp, C.POINTER(mk_server_stats)],
     "mk_server_destroy": [vp],
 }

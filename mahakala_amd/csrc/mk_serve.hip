@@ -24,6 +24,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <set>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -670,6 +671,7 @@ struct mk_server {
   struct Pending {
     int64_t id;
     int kind;
+    int priority;
     mk_image_request img;
     mk_shadow_request sh;
     std::vector<double> angles;
@@ -712,7 +714,11 @@ struct mk_server {
   bool want_shadow = false;          // a bisection waits for a launch that serves it
   std::chrono::steady_clock::time_point launch_t0;
   std::map<std::pair<int, double>, std::vector<int>> orders;  // tile orders by (resolution, fov)
-  std::deque<Pending> waiting;
+  std::deque<Pending> waiting;         // by priority, then submit order
+  Pending running{};                   // the request the idle worker runs itself
+  int64_t running_id = 0;              // its id while it runs, else 0
+  std::set<int64_t> cancelled;         // ids removed from the queue (the newest kCancelledKept)
+  std::deque<int64_t> cancelled_order;
   std::map<int, Flight> flight;
   std::vector<int> free_rids;
   std::vector<std::pair<int64_t, double>> unpolled;
@@ -734,6 +740,15 @@ size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 constexpr size_t kMaxUnpolled = size_t(1) << 20;  // completions kept for mk_server_poll
 constexpr int kDirectShadowAngles = 1024;         // bisections an idle server runs itself
 constexpr int kDirectImageRes = 512;              // and images, when one is all it has
+constexpr size_t kCancelledKept = size_t(1) << 16; // cancelled ids remembered for mk_server_wait
+
+// Queues a request behind every queued one of the same or a higher priority
+// (caller holds mu).
+void enqueue(mk_server* s, mk_server::Pending&& p) {
+  auto it = s->waiting.end();
+  while (it != s->waiting.begin() && std::prev(it)->priority < p.priority) --it;
+  s->waiting.insert(it, std::move(p));
+}
 
 bool arena_alloc(mk_server* s, size_t bytes, size_t* off) {
   for (auto it = s->free_blocks.begin(); it != s->free_blocks.end(); ++it) {
@@ -1034,18 +1049,19 @@ bool direct_next(mk_server* s) {
          p.img.resolution <= kDirectImageRes;
 }
 
-// Runs the request at the head of the queue, which direct_next chose,
-// without holding mu; it stays queued until its result is in its output, so
-// a wait on it does not return early.  Only the worker removes requests from
-// the queue, and a deque keeps references to its elements as others append.
+// Runs the request at the head of the queue, which direct_next chose: it
+// moves to the running slot, where a wait on it still finds it, and mu is
+// released while the GPU works.  Only the worker touches the slot.
 bool serve_direct(mk_server* s) {
-  const mk_server::Pending* p;
   const mk_snapshot* snap = nullptr;
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    p = &s->waiting.front();
-    if (p->kind == kKindImage) snap = s->snaps[p->img.snapshot];
+    s->running = std::move(s->waiting.front());
+    s->waiting.pop_front();
+    s->running_id = s->running.id;
+    if (s->running.kind == kKindImage) snap = s->snaps[s->running.img.snapshot];
   }
+  const mk_server::Pending* p = &s->running;
   long items = 0;
   SRV_HIP_UNLOCKED(hipEventRecord(s->ev0, s->stream));
   if (p->kind == kKindImage) {
@@ -1080,7 +1096,7 @@ bool serve_direct(mk_server* s) {
   ++s->st.admitted;
   ++s->st.completed;
   s->st.items_done += items;
-  s->waiting.pop_front();
+  s->running_id = 0;
   s->cv_done.notify_all();
   return true;
 }
@@ -1535,6 +1551,7 @@ int mk_server_submit_image(mk_server* s, const mk_image_request* r, double* h_ou
   mk_server::Pending p{};
   p.id = s->next_id++;
   p.kind = kKindImage;
+  p.priority = r->priority;
   p.img = *r;
   p.h_out = h_out;
   p.bytes = image_bytes(r->resolution);
@@ -1543,9 +1560,9 @@ int mk_server_submit_image(mk_server* s, const mk_image_request* r, double* h_ou
     return MK_ERR_NOMEM;
   }
   p.t0 = Clock::now();
-  s->waiting.push_back(std::move(p));
+  *id = p.id;
+  enqueue(s, std::move(p));
   ++s->st.submitted;
-  *id = s->waiting.back().id;
   s->cv_work.notify_one();
   return MK_OK;
 }
@@ -1558,6 +1575,7 @@ int mk_server_submit_shadow(mk_server* s, const mk_shadow_request* r, double* h_
   mk_server::Pending p{};
   p.id = s->next_id++;
   p.kind = kKindShadow;
+  p.priority = r->priority;
   p.sh = *r;
   p.angles.assign(r->h_angles, r->h_angles + r->n_angles);
   p.sh.h_angles = nullptr;
@@ -1568,10 +1586,32 @@ int mk_server_submit_shadow(mk_server* s, const mk_shadow_request* r, double* h_
     return MK_ERR_NOMEM;
   }
   p.t0 = Clock::now();
-  s->waiting.push_back(std::move(p));
+  *id = p.id;
+  enqueue(s, std::move(p));
   ++s->st.submitted;
-  *id = s->waiting.back().id;
   s->cv_work.notify_one();
+  return MK_OK;
+}
+
+int mk_server_cancel(mk_server* s, int64_t id, int32_t* h_cancelled) {
+  MK_REQUIRE(s && h_cancelled, "mk_server_cancel: null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  MK_REQUIRE(id > 0 && id < s->next_id, "mk_server_cancel: unknown request id %lld", (long long)id);
+  *h_cancelled = 0;
+  for (auto it = s->waiting.begin(); it != s->waiting.end(); ++it) {
+    if (it->id != id) continue;
+    s->waiting.erase(it);
+    if (s->cancelled_order.size() >= kCancelledKept) {
+      s->cancelled.erase(s->cancelled_order.front());
+      s->cancelled_order.pop_front();
+    }
+    s->cancelled.insert(id);
+    s->cancelled_order.push_back(id);
+    *h_cancelled = 1;
+    s->cv_work.notify_one();
+    s->cv_done.notify_all();
+    break;
+  }
   return MK_OK;
 }
 
@@ -1599,7 +1639,8 @@ int mk_server_wait(mk_server* s, int64_t id, double timeout_s) {
   // per-request record outlives the request
   auto ready = [&] {
     if (s->status != MK_OK) return true;
-    if (id < 0) return s->waiting.empty() && s->flight.empty();
+    if (id < 0) return s->waiting.empty() && s->flight.empty() && s->running_id == 0;
+    if (s->running_id == id) return false;
     for (const auto& p : s->waiting)
       if (p.id == id) return false;
     for (const auto& kv : s->flight)
@@ -1610,7 +1651,12 @@ int mk_server_wait(mk_server* s, int64_t id, double timeout_s) {
     set_error("mk_server_wait: timed out after %.3f s", timeout_s);
     return MK_ERR_TIMEOUT;
   }
-  return server_error(s);
+  if (int rc = server_error(s)) return rc;
+  if (id > 0 && s->cancelled.count(id)) {
+    set_error("mk_server_wait: request %lld was cancelled", (long long)id);
+    return MK_ERR_STATE;
+  }
+  return MK_OK;
 }
 
 int mk_server_get_stats(mk_server* s, mk_server_stats* out) {
@@ -1618,7 +1664,7 @@ int mk_server_get_stats(mk_server* s, mk_server_stats* out) {
   std::lock_guard<std::mutex> lk(s->mu);
   *out = s->st;
   out->queued = (int64_t)s->waiting.size();
-  out->in_flight = (int64_t)s->flight.size();
+  out->in_flight = (int64_t)s->flight.size() + (s->running_id != 0 ? 1 : 0);
   out->hbm_in_use = s->in_use;
   out->lanes = s->L;
   out->steps_per_launch = s->M;

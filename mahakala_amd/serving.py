@@ -67,10 +67,13 @@ class Server:
 
     def submit_image(self, snapshot, bhspin, inclination=60, distance=1000, fov=20, resolution=160,
                      observing_frequency=230.e9, mass_scale=1.e26, M_bh=6.2e9 * Msun, r_high=40,
-                     fluid_gamma=13 / 9, max_nsteps=10000):
-        """Queues an image request (make_image's parameters); returns its id."""
+                     fluid_gamma=13 / 9, max_nsteps=10000, priority=0):
+        """Queues an image request (make_image's parameters); returns its id.
+        A higher priority is admitted before queued requests of lower
+        priority; equal priorities keep submit order."""
         res = int(resolution)
-        req = _lib.mk_image_request(int(snapshot), res, int(max_nsteps), 0, float(bhspin), float(inclination),
+        req = _lib.mk_image_request(int(snapshot), res, int(max_nsteps), int(priority), float(bhspin),
+                                    float(inclination),
                                     float(distance), float(fov), float(observing_frequency), float(mass_scale),
                                     float(M_bh), float(r_high), float(fluid_gamma))
         out = np.empty((res, res), dtype=np.float64)
@@ -81,12 +84,13 @@ class Server:
         return rid.value
 
     def submit_shadow(self, bhspin, inclination, angles, distance=1000.0, max_steps=2000, error_allowed=1e-3,
-                      max_it=40):
+                      max_it=40, priority=0):
         """Queues a shadow bisection (find_shadow_bisection_angles); returns
-        its id.  The result is the radius per angle."""
+        its id.  The result is the radius per angle.  priority: as for
+        images."""
         ang = np.ascontiguousarray(angles, dtype=np.float64).ravel()
         req = _lib.mk_shadow_request(float(bhspin), float(inclination), float(distance), float(error_allowed),
-                                     ang.size, int(max_steps), int(max_it), 0, ang.ctypes.data)
+                                     ang.size, int(max_steps), int(max_it), int(priority), ang.ctypes.data)
         out = np.empty(ang.size, dtype=np.float64)
         rid = C.c_int64()
         with self._mu:
@@ -115,6 +119,16 @@ class Server:
         self.wait(rid, timeout)
         with self._mu:
             return self._out.pop(rid)
+
+    def cancel(self, rid):
+        """Removes request ``rid`` if it is still queued: True, and a later
+        result() raises MkError; False when it already runs or is done."""
+        done = C.c_int32()
+        _lib.check(_lib.lib().mk_server_cancel(self._h, int(rid), C.byref(done)))
+        if done.value:
+            with self._mu:
+                self._out.pop(rid, None)
+        return bool(done.value)
 
     def stats(self):
         st = _lib.mk_server_stats()

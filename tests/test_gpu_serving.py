@@ -171,6 +171,44 @@ def test_idle_server_renders_a_lone_small_image_itself(models, persistent):
     assert st["completed"] == 6 and st["in_flight"] == 0 and st["iterations"] >= 1
 
 
+@pytest.mark.parametrize("persistent", [False, True])
+def test_priorities_and_cancellation(models, persistent):
+    """With an output budget for one request at a time, queued requests are
+    admitted by priority, then in submit order; a cancelled queued request is
+    never run and a wait on it reports the cancellation; a request that is
+    running or done cannot be cancelled.  Every served image is mk_render's."""
+    res = 48
+    ang = np.linspace(0, 2 * np.pi, 1024, endpoint=False)
+    budget = 3 * ang.size * 8 + 1024         # the bisection's staging; one 48^2 image fits, two do not
+    assert res * res * 8 + 256 <= budget < 2 * (res * res * 8 + 256)
+    ref = _render(models[0], res, 0.5, 30.0, 230e9, "mixed")
+    with Server("mixed", hbm_budget=budget, persistent=persistent, shadow_in_launch=True,
+                image_in_launch=True) as srv:
+        sid = srv.add_snapshot(models[0])
+        a = srv.submit_shadow(0.9, 60.0, ang)                           # holds the budget for ~30 ms
+        t_wait = time.perf_counter() + 30.0
+        while srv.stats()["admitted"] < 1:                               # a is in flight
+            assert time.perf_counter() < t_wait, "the bisection was not admitted"
+            time.sleep(0.0005)
+        kw = _img_kw(res, 0.5, 30.0, 230e9)
+        b = srv.submit_image(sid, **kw)
+        c = srv.submit_image(sid, **kw)
+        d = srv.submit_image(sid, priority=5, **kw)
+        e = srv.submit_image(sid, **kw)
+        assert srv.cancel(c) is True
+        for rid in (b, d, e):
+            np.testing.assert_array_equal(srv.result(rid, timeout=120), ref)
+        srv.result(a, timeout=120)
+        with pytest.raises(_lib.MkError, match="cancelled"):
+            srv.wait(c, timeout=1.0)
+        assert srv.cancel(a) is False and srv.cancel(b) is False  # done
+        order = [i for i, _ in srv.poll()]
+        st = srv.stats()
+    print(f"[serving] completion order {order} (a={a}, b={b}, c={c} cancelled, d={d} priority 5, e={e})")
+    assert order == [a, d, b, e]
+    assert st["submitted"] == 5 and st["completed"] == 4 and st["queued"] == 0 and st["in_flight"] == 0
+
+
 def test_persistent_launch_ends_when_idle_and_resumes_its_tickets():
     """The persistent launch ends after the server has been idle; its waves
     leave holding unpublished tickets, and the next launch resumes at the
