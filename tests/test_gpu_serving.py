@@ -209,6 +209,69 @@ def test_priorities_and_cancellation(models, persistent):
     assert st["submitted"] == 5 and st["completed"] == 4 and st["queued"] == 0 and st["in_flight"] == 0
 
 
+@pytest.mark.parametrize("persistent", [False, True])
+def test_random_requests_priorities_and_cancellations(models, persistent):
+    """48 requests drawn at random (seed 7) from two submitting threads:
+    images of 8-136^2 on either snapshot at random spin, inclination and
+    frequency, and bisections of 1-300 angles, with random priorities, some
+    cancelled right after submission.  Every request that is not cancelled
+    (cancel() returned True) equals mk_render's image or mk_shadow_bisect's
+    radii bit for bit, whichever path served it (a launch, or the idle
+    worker), and the counts add up."""
+    import threading
+    rng = np.random.Generator(np.random.PCG64(7))
+    specs = []
+    for k in range(48):
+        if rng.random() < 0.7:
+            specs.append(("img", int(rng.integers(1, 18)) * 8, float(rng.uniform(-0.95, 0.95)),
+                          float(rng.uniform(5, 85)), float(rng.choice([86e9, 230e9, 345e9])), int(rng.integers(0, 2)),
+                          int(rng.integers(-2, 3)), bool(rng.random() < 0.2)))
+        else:
+            specs.append(("sh", int(rng.integers(1, 301)), float(rng.uniform(-0.95, 0.95)), float(rng.uniform(5, 85)),
+                          0.0, 0, int(rng.integers(-2, 3)), bool(rng.random() < 0.2)))
+    refs = {}
+    for k, (kind, n, a, inc, nu, snap, prio, _) in enumerate(specs):
+        if kind == "img":
+            refs[k] = _render(models[snap], n, a, inc, nu, "mixed")
+        else:
+            refs[k] = find_shadow_bisection_angles(a, inc, np.arange(n) / n * 2 * np.pi, max_steps=1500)
+    got, cancelled, errors = {}, set(), []
+    with Server("mixed", persistent=persistent) as srv:
+        sids = [srv.add_snapshot(m) for m in models]
+
+        def client(ks):
+            try:
+                mine = []
+                for k in ks:
+                    kind, n, a, inc, nu, snap, prio, cancel = specs[k]
+                    if kind == "img":
+                        rid = srv.submit_image(sids[snap], priority=prio, **_img_kw(n, a, inc, nu))
+                    else:
+                        rid = srv.submit_shadow(a, inc, np.arange(n) / n * 2 * np.pi, max_steps=1500, priority=prio)
+                    if cancel and srv.cancel(rid):
+                        cancelled.add(k)
+                    else:
+                        mine.append((k, rid))
+                for k, rid in mine:
+                    got[k] = srv.result(rid, timeout=120)
+            except Exception as e:  # reported below, on the main thread
+                errors.append(repr(e))
+
+        threads = [threading.Thread(target=client, args=(range(t, 48, 2),)) for t in range(2)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join(timeout=300)
+        srv.wait(-1, timeout=120)
+        st = srv.stats()
+    assert not errors, errors
+    print(f"[serving] random requests: {len(got)} served, {len(cancelled)} cancelled, {st['iterations']} launches")
+    assert sorted(set(got) | cancelled) == list(range(48)) and not (set(got) & cancelled)
+    for k, v in got.items():
+        np.testing.assert_array_equal(v, refs[k], err_msg=f"request {k}: {specs[k]}")
+    assert st["submitted"] == 48 and st["completed"] == len(got) and st["queued"] == 0 and st["in_flight"] == 0
+
+
 def test_persistent_launch_ends_when_idle_and_resumes_its_tickets():
     """The persistent launch ends after the server has been idle; its waves
     leave holding unpublished tickets, and the next launch resumes at the
