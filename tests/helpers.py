@@ -100,7 +100,7 @@ MSUN = 1.989e33
 
 
 def image_parity_vs_c(img, steps, cs, a, inc, res, nsteps=10000, nu=230e9, fov=20.0, rtol=1e-4, flux_rtol=1e-5,
-                      label=""):
+                      label="", dist=1000.0):
     """Plain FP64 GPU image and step counts against the C restatement of
     make_image (images.py:30-144): the same accepted-step count on every ray,
     the same finite/NaN pattern, every pixel with |I| above 1e-6 of the peak
@@ -110,7 +110,7 @@ def image_parity_vs_c(img, steps, cs, a, inc, res, nsteps=10000, nu=230e9, fov=2
     same bar).  Prints the counts.  A step-count mismatch is reported with the
     relative distance of the deciding candidate from the freeze threshold
     (geodesics.py:249-252), which is rounding-small when that is its cause."""
-    ref, rst, mg = cs.render(a, inc, 1000.0, fov, res, nsteps, 13. / 9, 40.0, 1e26, 6.2e9 * MSUN, nu, margins=True)
+    ref, rst, mg = cs.render(a, inc, dist, fov, res, nsteps, 13. / 9, 40.0, 1e26, 6.2e9 * MSUN, nu, margins=True)
     img = np.asarray(img).ravel()
     steps = np.asarray(steps).ravel()
     fin = np.isfinite(ref)

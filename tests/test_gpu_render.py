@@ -100,6 +100,25 @@ def test_extreme_views_plain_fp64_vs_c_restatement(torus, a, inc):
     assert abs(mixed.sum() - img.sum()) <= 1e-3 * img.sum()
 
 
+def test_random_views_plain_fp64_vs_c_restatement(torus):
+    """Six views drawn at random (seed 11): spin in (-0.99, 0.99), inclination
+    in (1, 179) deg, fov 8-40 M, camera at 200-1000 M, 86-690 GHz, 40^2
+    through the config-2 torus.  Plain FP64 against the C restatement: every
+    step count identical, every pixel within 1e-4."""
+    rng = np.random.Generator(np.random.PCG64(11))
+    cs = helpers.c_snapshot_from_model(torus)
+    res = 40
+    for k in range(6):
+        a, inc = float(rng.uniform(-0.99, 0.99)), float(rng.uniform(1, 179))
+        fov, dist = float(rng.uniform(8, 40)), float(rng.choice([200.0, 500.0, 1000.0]))
+        nu = float(rng.choice([86e9, 230e9, 345e9, 690e9]))
+        p = images.render_params(a, camera_inclination=inc, camera_distance=dist, fov=fov, resolution=res,
+                                 observing_frequency=nu)
+        img, steps = images.render(torus.device_snapshot, p, return_steps=True)
+        helpers.image_parity_vs_c(img, steps, cs, a, inc, res, nu=nu, fov=fov, dist=dist,
+                                  label=f"a={a:.3f} i={inc:.1f} fov={fov:.1f} d={dist:g} nu={nu:.3g}")
+
+
 def test_target_view_plain_fp64_vs_c_restatement():
     """The precision target's view at full size: 256^2, a = 0.9, i = 60 deg,
     the torus generated at a = 0.9 on 512 blocks of 32^3."""
