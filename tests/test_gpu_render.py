@@ -246,6 +246,34 @@ def test_kernel_forms_agree_bit_for_bit(torus, precision, res):
         np.testing.assert_array_equal(a, b)
 
 
+def test_kernel_forms_agree_at_random_views(torus):
+    """The four launch forms (classic from a counter, classic one slot per
+    tile, paired, triple) at four views drawn at random (seed 5: spin, an
+    inclination in (1, 179) deg, fov, frequency) and sizes of 40-72^2 (partial
+    tiles included), in plain FP64, mixed and mixed-zone, whose FP64 zone
+    decisions then fall where the view puts them: same image and step counts,
+    bit for bit."""
+    rng = np.random.Generator(np.random.PCG64(5))
+    for k in range(4):
+        a, inc = float(rng.uniform(-0.98, 0.98)), float(rng.uniform(1, 179))
+        fov, nu, res = float(rng.uniform(10, 30)), float(rng.choice([86e9, 230e9, 690e9])), int(rng.integers(40, 73))
+        for precision in ("plain64", "mixed", "mixed-zone"):
+            out = {}
+            for form in ("classic", "paired", "triple", "classic-static"):
+                p = images.render_params(a, camera_inclination=inc, fov=fov, resolution=res, observing_frequency=nu,
+                                         precision=precision, form=form.split("-")[0])
+                if form == "classic-static":
+                    p.flags |= MK_RENDER_STATIC_TILES
+                out[form] = images.render(torus.device_snapshot, p, return_steps=True)
+            base, sb = out["classic"]
+            for form in ("paired", "triple", "classic-static"):
+                img, st = out[form]
+                np.testing.assert_array_equal(st, sb, err_msg=f"{form} {precision} view {k}")
+                np.testing.assert_array_equal(img, base, err_msg=f"{form} {precision} view {k}")
+        print(f"[render] random view {k} (a={a:.3f} i={inc:.1f} fov={fov:.1f} nu={nu:.3g} {res}^2): "
+              f"four forms identical in three precisions")
+
+
 def test_classic_tile_counter_starts_from_zero_every_launch(torus):
     """The classic form's waves take tiles from a per-stream counter that the
     last wave of each launch resets: back-to-back launches on the null stream
