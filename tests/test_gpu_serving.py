@@ -202,6 +202,8 @@ def test_priorities_and_cancellation(models, persistent):
         with pytest.raises(_lib.MkError, match="cancelled"):
             srv.wait(c, timeout=1.0)
         assert srv.cancel(a) is False and srv.cancel(b) is False  # done
+        with pytest.raises(_lib.MkError, match="unknown request"):
+            srv.cancel(10 ** 6)
         order = [i for i, _ in srv.poll()]
         st = srv.stats()
     print(f"[serving] completion order {order} (a={a}, b={b}, c={c} cancelled, d={d} priority 5, e={e})")
