@@ -110,3 +110,40 @@ def test_athdf_library_exports_every_declared_symbol():
     exported = set(re.findall(r" T (mk_athdf_\w+)", out))
     assert not [s for s in declared if s not in exported]
     _athdf.lib()
+
+
+def test_header_is_plain_c():
+    """include/mahakala_hip.h is a C header (the boundary a cgo, JNI or ctypes
+    binding includes or restates): it compiles as strict C99 and as C++, and a
+    C program that uses its structs and flags links against the library."""
+    import shutil
+    import tempfile
+    if not shutil.which("gcc"):
+        pytest.skip("no gcc")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    inc = os.path.join(root, "include")
+    src = ("#include <stddef.h>\n#include <string.h>\n#include \"mahakala_hip.h\"\n"
+           "int main(void) {\n"
+           "  mk_render_params p;\n  mk_server_config cfg;\n  mk_image_request r;\n  mk_shadow_request sh;\n"
+           "  memset(&p, 0, sizeof p); memset(&cfg, 0, sizeof cfg); memset(&r, 0, sizeof r);\n"
+           "  memset(&sh, 0, sizeof sh);\n"
+           "  p.flags = MK_RENDER_CLASSIC | MK_RENDER_STATIC_TILES | MK_RENDER_ROW_MAJOR;\n"
+           "  cfg.flags = MK_SERVER_PERSISTENT | MK_SERVER_SHADOW_IN_LAUNCH | MK_SERVER_IMAGE_IN_LAUNCH;\n"
+           "  r.priority = 1; sh.priority = 2;\n"
+           "  return mk_version() > 0 && p.flags && cfg.flags && r.priority + sh.priority == 3 ? 0 : 1;\n"
+           "}\n")
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        with open(c, "w") as f:
+            f.write(src)
+        for cmd in (["gcc", "-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror", "-I", inc, "-c", c,
+                     "-o", os.path.join(d, "t.o")],
+                    ["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", inc, "-x", "c++", "-c", c,
+                     "-o", os.path.join(d, "tpp.o")]):
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            assert r.returncode == 0, r.stderr
+        lib_dir = os.path.dirname(_lib.LIB_PATH)
+        if os.path.exists(_lib.LIB_PATH):
+            r = subprocess.run(["gcc", "-o", os.path.join(d, "t"), os.path.join(d, "t.o"), "-L", lib_dir,
+                                "-lmahakala_hip", f"-Wl,-rpath,{lib_dir}"], capture_output=True, text=True)
+            assert r.returncode == 0, r.stderr
