@@ -56,10 +56,24 @@ template <> struct Math<double> {
     return x == 0.0 ? x : g;
   }
   static __device__ __forceinline__ double sqrt_ieee(double x) { return __builtin_sqrt(x); }
-  // sqrt(x) and 1/sqrt(x)
+  // sqrt(x) and 1/sqrt(x) from one reciprocal square root: two Goldschmidt
+  // steps carry g -> sqrt(x) and h -> 1/(2 sqrt(x)) from the estimate's
+  // 2^-24 to about 1.5 ulp, one residual correction rounds g as sqrt()
+  // does (the same bits), and 1/sqrt(x) = 2h (within 1.5 ulp, as a refined
+  // reciprocal of sqrt(x) is) without a second quarter-rate estimate
   static __device__ __forceinline__ void sqrt_rcp(double x, double& s, double& is) {
-    s = sqrt(x);
-    is = rcp(s);
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double r = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    r = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    const double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    s = x == 0.0 ? x : g;
+    is = 2.0 * h;
   }
   static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
   static __device__ __forceinline__ double pow(double x, double y) { return ::pow(x, y); }
