@@ -68,7 +68,9 @@ template <> struct Math<double> {
   // steps carry g -> sqrt(x) and h -> 1/(2 sqrt(x)) from the estimate's
   // 2^-24 to about 1.5 ulp, one residual correction rounds g as sqrt()
   // does (the same bits), and 1/sqrt(x) = 2h (within 1.5 ulp, as a refined
-  // reciprocal of sqrt(x) is) without a second quarter-rate estimate
+  // reciprocal of sqrt(x) is) without a second quarter-rate estimate.  For
+  // x = 0 (on the right-hand side only at the ring singularity, r = 0) s is
+  // NaN, not 0; the right-hand side is NaN there either way (is = inf)
   static __device__ __forceinline__ void sqrt_rcp(double x, double& s, double& is) {
     const double y = __builtin_amdgcn_rsq(x);
     double g = x * y, h = 0.5 * y;
@@ -79,8 +81,7 @@ template <> struct Math<double> {
     g = __builtin_fma(g, r, g);
     h = __builtin_fma(h, r, h);
     const double d = __builtin_fma(-g, g, x);
-    g = __builtin_fma(d, h, g);
-    s = x == 0.0 ? x : g;
+    s = __builtin_fma(d, h, g);
     is = 2.0 * h;
   }
   static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
