@@ -169,25 +169,30 @@ __device__ __forceinline__ void ks_accel(T x, T y, T z, T v0, T v1, T v2, T v3, 
   M::sqrt_rcp(rr, r, ir);
   T iden, s;
   M::rcp2(rr * rr + aa * zz, rr + aa, iden, s);
-  const T f = (T(2) * rr * r) * iden;
-  const T l1 = (r * x + a * y) * s;
-  const T l2 = (r * y - a * x) * s;
+  // products shared below: t2 = 2 r^2/den (f = t2 r, and kappa's second
+  // term), rs = r s and as = a s (l1, l2, W.v, the rotation term of D - W),
+  // hi = 1/(2 r^2) (A3, kappa); 2 a^2 is uniform
+  const T t2 = T(2) * rr * iden;
+  const T f = t2 * r;
+  const T rs = r * s, as = a * s;
+  const T l1 = rs * x + as * y;
+  const T l2 = rs * y - as * x;
   const T l3 = z * ir;
   // gradient of r^2, and A
   const T g = rr * iQ;
   const T d1 = x * g, d2 = y * g, d3 = z * (rr + aa) * iQ;
-  const T h = T(0.5) * ir;
-  const T A1 = s * (h * x - l1), A2 = s * (h * y - l2), A3 = -l3 * h * ir;
-  const T kap = f * (T(3) * h * ir - T(2) * rr * iden);
-  const T zeta = T(2) * f * aa * z * iden;
+  const T h = T(0.5) * ir, hi = h * ir;
+  const T A1 = s * (h * x - l1), A2 = s * (h * y - l2), A3 = -l3 * hi;
+  const T kap = f * (T(3) * hi - t2);
+  const T zeta = f * iden * (T(2) * aa) * z;
   const T L = v0 + l1 * v1 + l2 * v2 + l3 * v3;
   const T dv = d1 * v1 + d2 * v2 + d3 * v3;
   const T Av = A1 * v1 + A2 * v2 + A3 * v3;
-  const T Wv = dv * Av + s * r * (v1 * v1 + v2 * v2) + ir * (v3 * v3);
+  const T Wv = dv * Av + rs * (v1 * v1 + v2 * v2) + ir * (v3 * v3);
   const T P = L * (kap * dv - zeta * v3) + f * Wv;
   const T fL = f * L, hL2 = T(0.5) * L * L;
   const T mu = hL2 * kap + fL * Av, nu = fL * dv;
-  const T sa2 = T(2) * s * a * fL;
+  const T sa2 = T(2) * as * fL;
   const T c1 = mu * d1 - nu * A1 - sa2 * v2 - l1 * P;
   const T c2 = mu * d2 - nu * A2 + sa2 * v1 - l2 * P;
   const T c3 = mu * d3 - nu * A3 - l3 * P - hL2 * zeta;
