@@ -497,11 +497,11 @@ def main():
         "value": round(value, 1), "unit": "rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * wall / args.steps, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32+fp64" if head == "mixed" else "fp64", "data": "synthetic",
-        "config": {"workload": "256x256 image, a=0.5, i=30deg, 230 GHz, analytic torus on 512 meshblocks of 32^3 "
+        "config": {"workload": f"{res}x{res} image, a=0.5, i=30deg, 230 GHz, analytic torus on 512 meshblocks of 32^3 "
                                "FP32 cells over +-40 M",
                    "resolution": res, "rays_per_step": n, "precision": head, "max_nsteps": 10000},
         "image_launch_ms": {"p50": round(float(np.percentile(ms_head, 50)), 4),
-                            "p99": round(float(np.percentile(ms_head, 99)), 4), "what": "one 256^2 image per launch"},
+                            "p99": round(float(np.percentile(ms_head, 99)), 4), "what": f"one {res}^2 image per launch"},
         "roofline": roofline,
         "accuracy_mixed_vs_plain": {k: (round(v, 9) if isinstance(v, float) else v) for k, v in acc.items()},
         "accepted_steps": int(total_steps), "mean_steps_per_ray": round(total_steps / n, 2),
