@@ -40,6 +40,34 @@ def test_camera_equator_and_pixels():
     np.testing.assert_allclose(gpu, onp.initial_condition(rx, rv, 0.3), rtol=1e-13, atol=1e-12)
 
 
+@pytest.mark.parametrize("a", [0.0, 0.5, 0.94, -0.98])
+def test_fp64_rhs_and_rk4_step_near_ulp_level(a):
+    """The FP64 right-hand side and RK4 step (geodesics.py:294-336) against the
+    C restatement's IEEE evaluation of the same closed form, on 4000 states
+    from r = 1.6 M to 1200 M: the device's square roots and reciprocals come
+    from hardware estimates refined to within 2 ulp.  Measured on MI355X: each
+    state's largest deviation is within 8.6e-15 of its largest component for
+    the right-hand side (the C and NumPy restatements differ by up to 4e-15 on
+    these states) and within 2.3e-16 for the step."""
+    rng = np.random.Generator(np.random.PCG64(7))
+    n = 4000
+    r = np.exp(rng.uniform(np.log(1.6), np.log(1200), n))
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    x = np.zeros((n, 4))
+    x[:, 1:] = u * r[:, None]
+    s = np.concatenate([x, rng.normal(size=(n, 4))], 1)
+    ref = c_oracle.rhs(s, a)
+    err = np.abs(np.asarray(G.rhs(s, a)) - ref).max(1) / np.abs(ref).max(1)
+    dt = -0.025 * (r - (1 + np.sqrt(1 - a * a)))
+    ref4 = c_oracle.rk4(s, dt, a)
+    err4 = np.abs(np.asarray(G.RK4_gen(s, dt, a)) - ref4).max(1) / np.abs(ref4).max(1)
+    print(f"[rhs a={a}] per-state max rel: rhs {err.max():.2e} (median {np.median(err):.1e}), "
+          f"rk4 {err4.max():.2e} (median {np.median(err4):.1e})", flush=True)
+    assert err.max() <= 3e-14
+    assert err4.max() <= 2e-15
+
+
 def test_metric_rhs_rk4():
     rng = np.random.Generator(np.random.PCG64(5))
     x = np.zeros((200, 4))
