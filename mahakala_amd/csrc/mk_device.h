@@ -307,10 +307,17 @@ __device__ __forceinline__ double step_rule_fast(double r, double rEH, double in
 }
 
 // geodesics.py:249-252: dt = -(r - r_EH)/div, zeroed on NaN, |dt| div < tol
-// or |dt| div > 1500
+// or |dt| div > 1500.  The quotient is the IEEE one: with y = 1/div rounded
+// (loop-invariant, so computed once) and q = num y, the residual num - q div
+// is exact in one FMA and q + residual y rounds to num/div (Markstein's
+// theorem; no overflow or underflow here).  A zero num gives +0 where the
+// division gives -0, and both are frozen below.
 __device__ __forceinline__ double step_rule(double r, double rEH, double div, double tol) {
 #pragma clang fp contract(off)
-  const double dt = -(r - rEH) / div;
+  const double num = -(r - rEH);
+  const double y = 1.0 / div;
+  const double q = num * y;
+  const double dt = __builtin_fma(__builtin_fma(-q, div, num), y, q);
   const double m = __builtin_fabs(dt) * div;
   if (dt != dt || m < tol || m > 1500.0) return 0.0;
   return dt;
