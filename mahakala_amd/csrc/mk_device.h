@@ -280,12 +280,17 @@ __device__ __forceinline__ void rk4_step_mixed(const double s[8], double dt, flo
   for (int i = 1; i < 8; ++i) out[i] = s[i] + (double)(h6 * (sum[i] + F[i]));
 }
 
-// geodesics.py:284-291
+// geodesics.py:284-291.  The square roots are IEEE ones: the compiler's
+// FP64 sqrt is this same rsq estimate, Goldschmidt step and two residual
+// corrections (Math<double>::sqrt), wrapped in a rescaling for arguments
+// below 2^-767 and a pass-through for 0 and inf; the arguments here are 0 or
+// far above 2^-767, and an infinite one freezes the ray either way.
 __device__ __forceinline__ double radius_cal(double x, double y, double z, double a) {
 #pragma clang fp contract(off)
-  const double R = __builtin_sqrt(x * x + y * y + z * z);
+  using M = Math<double>;
+  const double R = M::sqrt(x * x + y * y + z * z);
   const double R2 = R * R, aa = a * a;
-  return __builtin_sqrt((R2 - aa + __builtin_sqrt((R2 - aa) * (R2 - aa) + 4.0 * aa * (z * z))) / 2.0);
+  return M::sqrt((R2 - aa + M::sqrt((R2 - aa) * (R2 - aa) + 4.0 * aa * (z * z))) / 2.0);
 }
 
 // radius_cal in FP32 from the FP64 position (the mixed step rule: FP32
